@@ -1,0 +1,138 @@
+/*
+ * hiptree: MI355X-native tree-likelihood engine, C ABI.
+ *
+ * Every entry point takes plain pointers and sizes; device pointers are HIP
+ * device memory owned by the caller unless stated otherwise. Calls are
+ * asynchronous on the given stream and return an hx_status; nothing throws
+ * across this boundary. A handle (plan, server) is externally synchronised.
+ *
+ * Array conventions (row-major, innermost last):
+ *   T taxa, I = T-1 internal nodes, B = 2T-2 branches (branch b = node b),
+ *   S states, N site patterns, C rate categories, K samples.
+ *   postorder   int32 [I][3]   (node, left, right) in post-order, root last
+ *   mats        R [K][B][C][S][S]   transition matrices, R = float | double
+ *   freqs       R [K or 1][S]
+ *   props       R [K or 1][C]
+ *   weights     double [N]
+ *   tip codes   uint8 [T][N] with a table R [ncodes][S] of tip partial rows,
+ *               or dense tip partials R [T][N][S]
+ *
+ * Reference interfaces replaced (torchtree 1.0.2, torchtree/evolution/):
+ *   hx_loglik      -> calculate_treelikelihood (tree_likelihood.py:14-37),
+ *                     calculate_treelikelihood_discrete (40-75),
+ *                     ..._tip_states_discrete (78-131), ..._discrete_safe (134-183),
+ *                     ..._discrete_rescaled (186-221),
+ *                     ..._tip_states_discrete_rescaled (224-278)
+ *   hx_loglik_grad_mats -> the autograd backward of the functions above
+ *                     with respect to mats, freqs and props (the reference
+ *                     relies on torch autograd; there is no explicit code)
+ *   hx_pmatrix     -> SymmetricSubstitutionModel.p_t (substitution_model/abstract.py:57-79)
+ *                     and EmpiricalSubstitutionModel.p_t (general.py:318-331)
+ *   hx_pmatrix_grad -> autograd backward of p_t through torch.linalg.eigh
+ *                     (abstract.py:79), expressed with the exp divided differences
+ */
+#ifndef HIPTREE_H
+#define HIPTREE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* hx_stream_t; /* same object as hipStream_t */
+
+typedef enum {
+  HX_OK = 0,
+  HX_ERR_INVALID_ARGUMENT = 1,
+  HX_ERR_UNSUPPORTED = 2,
+  HX_ERR_HIP = 3,
+  HX_ERR_WORKSPACE_TOO_SMALL = 4,
+  HX_ERR_BAD_TOPOLOGY = 5,
+  HX_ERR_OUT_OF_MEMORY = 6,
+  HX_ERR_BUDGET = 7,
+  HX_ERR_NOT_FOUND = 8
+} hx_status;
+
+enum { HX_F32 = 0, HX_F64 = 1 };
+enum { HX_TIPS_CODES = 0, HX_TIPS_DENSE = 1 };
+
+/* Human-readable text for a status, plus the last error detail. */
+const char* hx_error_string(int status);
+const char* hx_last_error(void);
+int hx_version(void);
+
+/* ---------------------------------------------------------------- plans */
+/* A plan holds one topology: the post-order triples, the height-level
+ * schedule for the post-order pass and the depth-level schedule for the
+ * pre-order pass, copied once to device memory owned by the plan.
+ * Replaces the per-call Python loop over post_indexing
+ * (tree_likelihood.py:67-70) and AbstractTreeModel.update_traversals
+ * (tree_model.py:187-195). `postorder` is a host pointer. */
+typedef struct hx_plan_s* hx_plan_t;
+int hx_plan_create(int taxa, const int32_t* postorder, int device, hx_plan_t* plan);
+int hx_plan_destroy(hx_plan_t plan);
+/* levels: post-order height levels; depth: pre-order depth levels. */
+int hx_plan_info(hx_plan_t plan, int* post_levels, int* pre_levels, int* max_width);
+
+/* Device workspace needed by hx_loglik / hx_loglik_grad_mats for one call. */
+size_t hx_workspace_bytes(hx_plan_t plan, int S, int N, int C, int K, int dtype);
+
+/* --------------------------------------------------------- likelihood */
+/* Post-order pruning with exact power-of-two rescaling of every node,
+ * fused root reduction. Outputs (device, double):
+ *   out_loglik [K]      sum_n w_n log L_n
+ *   out_dfreqs [K][S]   d loglik / d freqs through the root (may be NULL)
+ *   out_dprops [K][C]   d loglik / d props (may be NULL)
+ * The workspace keeps the post-order partials; pass the same workspace to
+ * hx_loglik_grad_mats afterwards to get the gradient with respect to mats. */
+int hx_loglik(hx_plan_t plan, int S, int N, int C, int K, int dtype,
+              int tip_kind, const void* tips, const void* tip_table, int ncodes,
+              const double* weights, const void* mats,
+              const void* freqs, int freqs_batched,
+              const void* props, int props_batched,
+              void* workspace, size_t workspace_bytes,
+              double* out_loglik, double* out_dfreqs, double* out_dprops,
+              hx_stream_t stream);
+
+/* Pre-order adjoint over the partials left in `workspace` by hx_loglik
+ * (consumed: the partials are overwritten in place). Output:
+ *   out_dmats [K][B][C][S][S] double = d loglik / d mats. */
+int hx_loglik_grad_mats(hx_plan_t plan, int S, int N, int C, int K, int dtype,
+                        int tip_kind, const void* tips, const void* tip_table, int ncodes,
+                        const double* weights, const void* mats,
+                        const void* freqs, int freqs_batched,
+                        const void* props, int props_batched,
+                        void* workspace, size_t workspace_bytes,
+                        double* out_dmats, hx_stream_t stream);
+
+/* ------------------------------------------------------------ P matrices */
+/* mats[k][b][c] = L_e diag(exp(lambda_e * bls[k][b] * rates[k][c])) R_e with
+ * e = k if eigen_batched else 0. lambda [E][S], left [E][S][S] (Pi^-1/2 V),
+ * right [E][S][S] (V^T Pi^1/2), all double; bls [K][B], rates [K or 1][C]
+ * double; mats written in `dtype`. */
+int hx_pmatrix(int S, int K, int B, int C, int eigen_batched,
+               const double* lambda, const double* left, const double* right,
+               const double* bls, const double* rates, int rates_batched,
+               void* mats, int dtype, hx_stream_t stream);
+
+/* Chain rule from dmats = d loglik / d mats [K][B][C][S][S] (double) to
+ *   out_dbls  [K][B]      d loglik / d bls
+ *   out_drates[K][C]      d loglik / d rates (sum over branches)
+ *   out_dq    [E][S][S]   d loglik / d Q (normalised Q, summed over samples
+ *                         that share an eigensystem)
+ * using P = exp(Q tau) and the divided differences of exp on the spectrum
+ * (exact at repeated eigenvalues). `scratch` needs
+ * hx_pmatrix_grad_scratch_bytes(S, K, B) bytes of device memory. */
+size_t hx_pmatrix_grad_scratch_bytes(int S, int K, int B);
+int hx_pmatrix_grad(int S, int K, int B, int C, int eigen_batched,
+                    const double* lambda, const double* left, const double* right,
+                    const double* bls, const double* rates, int rates_batched,
+                    const double* dmats, double* out_dbls, double* out_drates,
+                    double* out_dq, void* scratch, hx_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HIPTREE_H */

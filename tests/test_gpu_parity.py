@@ -1,0 +1,158 @@
+"""HIP kernels against the FP64 oracle on identical seeded inputs.
+
+Tolerances: FP64 kernels relative 1e-12 (log-likelihood) and 1e-10
+(gradients); FP32-with-rescaling kernels relative 1e-6 (log-likelihood) and
+1e-5 (gradient, norm-wise), the targets of the engine.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import data as odata
+from oracle import models as omodels
+from oracle import pruning as opruning
+
+pytestmark = pytest.mark.gpu
+
+F64 = torch.float64
+
+
+def _flush(msg):
+    print(msg, flush=True)
+
+
+def oracle_model_loglik(wl, bls, exch, freqs, rates, props, model="gtr"):
+    """Oracle logL [K] of a synthetic workload with autograd (reference method)."""
+    if model == "gtr":
+        q = omodels.gtr_q(exch, freqs)
+    else:
+        q = omodels.q_from_exchangeabilities(exch, freqs)
+    mats = omodels.p_t_eigh(q, freqs, bls.unsqueeze(-1) * rates.reshape(1, 1, -1))
+    tips = wl.tip_partials()
+    post = [tuple(int(x) for x in r) for r in wl.topo.postorder]
+    lp = opruning.treelik_discrete_rescaled(tips + [None] * (wl.T - 1), wl.weights, post, mats,
+                                            freqs.reshape(1, -1), props.reshape(-1, 1, 1))
+    return lp.reshape(-1)
+
+
+def test_tiny_dropin_fp64(gpu, tiny_newick, tiny_fasta):
+    _flush("case: tiny drop-in calculate_treelikelihood fp64")
+    from torchtree_amd import likelihood as hl
+
+    tree = odata.Tree(tiny_newick)
+    seqs = dict(odata.read_fasta(tiny_fasta))
+    tips, weights = odata.compress_alignment([seqs[t] for t in tree.taxa], odata.nucleotide_partial)
+    bl = torch.tensor(tree.edge_lengths(), dtype=F64)
+    mats = omodels.jc69_p_t(bl)
+    freqs = torch.full((1, 4), 0.25, dtype=F64)
+    ref = opruning.treelik(tips + [None] * (tree.T - 1), weights, tree.postorder, mats, freqs)
+    got = hl.calculate_treelikelihood(tips + [None] * (tree.T - 1), weights, tree.postorder,
+                                      mats.to(gpu), freqs.to(gpu))
+    assert got.shape == ref.shape
+    assert abs(float(got) - (-83.329016)) < 1e-5
+    assert math.isclose(float(got), float(ref), rel_tol=1e-12)
+
+
+def test_flu_dropin_fp64_and_fp32(gpu, flu_newick, flu_fasta):
+    _flush("case: fluA drop-in discrete_rescaled fp64/fp32")
+    from torchtree_amd import likelihood as hl
+    from tests.test_oracle_pruning import flu_setup
+
+    tree, tips, weights, bl = flu_setup(flu_newick, flu_fasta)
+    rates, props = omodels.weibull_rates_probs(torch.tensor([0.1], dtype=F64), 4)
+    mats = omodels.jc69_p_t(bl.reshape(-1, 1) * rates.reshape(1, -1))
+    freqs = torch.full((1, 4), 0.25, dtype=F64)
+    pr = props.reshape(-1, 1, 1)
+    partials = tips + [None] * (tree.T - 1)
+    ref = opruning.treelik_discrete_rescaled(partials, weights, tree.postorder, mats, freqs, pr)
+    got = hl.calculate_treelikelihood_discrete_rescaled(partials, weights, tree.postorder,
+                                                        mats.to(gpu), freqs.to(gpu), pr.to(gpu))
+    assert math.isclose(float(got), float(ref), rel_tol=1e-12)
+    assert abs(float(got) - (-4618.2062529058)) < 4618.2 * 1e-5
+    got32 = hl.calculate_treelikelihood_discrete(partials, weights, tree.postorder,
+                                                 mats.float().to(gpu), freqs.float().to(gpu),
+                                                 pr.float().to(gpu))
+    assert math.isclose(float(got32), float(ref), rel_tol=1e-6)
+
+
+def test_config1_fp64(gpu):
+    _flush("case: config1 JC69 64 taxa fp64 via engine")
+    from torchtree_amd import synthetic
+    from torchtree_amd.engine import TreeJob
+
+    wl = synthetic.config1()
+    post = [tuple(int(x) for x in r) for r in wl.topo.postorder]
+    job = TreeJob(post, list(wl.codes), wl.weights, 4, dtype=F64, device=gpu, tip_kind="states")
+    got = job.loglik(wl.bls.to(gpu), wl.rates.reshape(1, -1), wl.props.reshape(1, -1),
+                     wl.q_norm, wl.freqs)
+    ref = oracle_model_loglik(wl, wl.bls, wl.exch, wl.freqs, wl.rates, wl.props)
+    assert math.isclose(float(got[0]), float(ref[0]), rel_tol=1e-12)
+    job32 = TreeJob(post, list(wl.codes), wl.weights, 4, dtype=torch.float32, device=gpu,
+                    tip_kind="states")
+    got32 = job32.loglik(wl.bls.to(gpu), wl.rates.reshape(1, -1), wl.props.reshape(1, -1),
+                         wl.q_norm, wl.freqs)
+    assert math.isclose(float(got32[0]), float(ref[0]), rel_tol=1e-6)
+
+
+@pytest.mark.parametrize("dtype,rtol_l,rtol_g", [(F64, 1e-12, 1e-10), (torch.float32, 1e-6, 1e-5)])
+def test_dropin_grad_mats(gpu, dtype, rtol_l, rtol_g):
+    _flush(f"case: drop-in d/dmats {dtype}")
+    from torchtree_amd import likelihood as hl
+    from torchtree_amd import synthetic
+
+    wl = synthetic.make_workload(24, 300, 3, model="gtr", seed=11, bl_mean=0.1, gap_fraction=0.05)
+    post = [tuple(int(x) for x in r) for r in wl.topo.postorder]
+    tips = wl.tip_partials()
+    q = omodels.gtr_q(wl.exch, wl.freqs)
+    tau = wl.bls.unsqueeze(-1) * wl.rates.reshape(1, 1, -1)
+    mats = omodels.p_t_eigh(q, wl.freqs, tau).detach().requires_grad_(True)
+    freqs = wl.freqs.reshape(1, -1).clone().requires_grad_(True)
+    props = wl.props.reshape(-1, 1, 1).clone().requires_grad_(True)
+    ref = opruning.treelik_discrete_rescaled(tips + [None] * (wl.T - 1), wl.weights, post, mats,
+                                             freqs, props)
+    ref.sum().backward()
+    m2 = mats.detach().to(gpu, dtype).requires_grad_(True)
+    f2 = freqs.detach().to(gpu, dtype).requires_grad_(True)
+    p2 = props.detach().to(gpu, dtype).requires_grad_(True)
+    got = hl.calculate_treelikelihood_discrete(tips + [None] * (wl.T - 1), wl.weights, post, m2, f2, p2)
+    assert got.shape == ref.shape
+    rel = (got.double().cpu() - ref.detach()).abs() / ref.detach().abs()
+    assert float(rel.max()) < rtol_l
+    got.sum().backward()
+    for a, b in ((m2.grad, mats.grad), (f2.grad, freqs.grad), (p2.grad, props.grad)):
+        a = a.double().cpu()
+        err = float((a - b).norm() / b.norm())
+        assert err < rtol_g, err
+
+
+@pytest.mark.parametrize("dtype,rtol_l,rtol_g", [(F64, 1e-12, 1e-9), (torch.float32, 1e-6, 1e-5)])
+def test_engine_grad_model(gpu, dtype, rtol_l, rtol_g):
+    _flush(f"case: engine GTR+W4 full gradient {dtype}")
+    from torchtree_amd import models as hm
+    from torchtree_amd import synthetic
+    from torchtree_amd.engine import TreeJob
+
+    wl = synthetic.make_workload(32, 500, 4, model="gtr", seed=12, bl_mean=0.05)
+    post = [tuple(int(x) for x in r) for r in wl.topo.postorder]
+    # oracle (reference method, autograd through eigh)
+    bls = wl.bls.clone().requires_grad_(True)
+    exch = wl.exch.clone().requires_grad_(True)
+    freqs = wl.freqs.clone().requires_grad_(True)
+    ref = oracle_model_loglik(wl, bls, exch, freqs, wl.rates, wl.props)
+    ref.sum().backward()
+    # engine
+    job = TreeJob(post, list(wl.codes), wl.weights, 4, dtype=dtype, device=gpu, tip_kind="states")
+    bls2 = wl.bls.clone().to(gpu).requires_grad_(True)
+    exch2 = wl.exch.clone().requires_grad_(True)
+    freqs2 = wl.freqs.clone().requires_grad_(True)
+    qn = hm.normalize(hm.gtr_q(exch2, freqs2), freqs2)
+    got = job.loglik(bls2, wl.rates.reshape(1, -1), wl.props.reshape(1, -1), qn, freqs2)
+    rel = (got.cpu() - ref.detach()).abs() / ref.detach().abs()
+    assert float(rel.max()) < rtol_l
+    got.sum().backward()
+    for a, b, name in ((bls2.grad.cpu(), bls.grad, "bls"), (exch2.grad, exch.grad, "exch"),
+                       (freqs2.grad, freqs.grad, "freqs")):
+        err = float((a - b).norm() / b.norm())
+        assert err < rtol_g, (name, err)

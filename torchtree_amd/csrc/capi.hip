@@ -1,0 +1,158 @@
+// C ABI: status strings, topology plans and workspace sizing.
+#include "common.hpp"
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+namespace hx {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+const std::string& get_error() { return g_last_error; }
+
+}  // namespace hx
+
+extern "C" {
+
+const char* hx_error_string(int status) {
+  switch (status) {
+    case HX_OK: return "ok";
+    case HX_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case HX_ERR_UNSUPPORTED: return "unsupported configuration";
+    case HX_ERR_HIP: return "HIP runtime error";
+    case HX_ERR_WORKSPACE_TOO_SMALL: return "workspace too small";
+    case HX_ERR_BAD_TOPOLOGY: return "bad topology";
+    case HX_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case HX_ERR_BUDGET: return "request exceeds the memory budget";
+    case HX_ERR_NOT_FOUND: return "not found";
+    default: return "unknown status";
+  }
+}
+
+const char* hx_last_error(void) { return hx::get_error().c_str(); }
+
+int hx_version(void) { return 100; }
+
+// Builds the post-order height levels and pre-order depth levels of a rooted
+// binary tree given as (node, left, right) triples in post-order, the
+// convention of AbstractTreeModel.update_traversals (tree_model.py:187-195):
+// tips 0..T-1, internal nodes T..2T-2, root 2T-2 last.
+int hx_plan_create(int taxa, const int32_t* postorder, int device, hx_plan_t* out) {
+  HX_REQUIRE(out != nullptr && postorder != nullptr, HX_ERR_INVALID_ARGUMENT, "null argument");
+  HX_REQUIRE(taxa >= 2, HX_ERR_BAD_TOPOLOGY, "need at least two taxa");
+  const int T = taxa, I = T - 1, nn = 2 * T - 1;
+  std::vector<int32_t> kids(2 * nn, -1);
+  std::vector<int> parent(nn, -1);
+  std::vector<char> seen_parent(nn, 0);
+  for (int r = 0; r < I; ++r) {
+    int v = postorder[3 * r], a = postorder[3 * r + 1], b = postorder[3 * r + 2];
+    if (v < T || v >= nn || a < 0 || a >= nn || b < 0 || b >= nn || a == b) {
+      hx::set_error("triple " + std::to_string(r) + " out of range");
+      return HX_ERR_BAD_TOPOLOGY;
+    }
+    if (seen_parent[v] || parent[a] >= 0 || parent[b] >= 0) {
+      hx::set_error("node repeated in post-order triples");
+      return HX_ERR_BAD_TOPOLOGY;
+    }
+    if ((a >= T && !seen_parent[a]) || (b >= T && !seen_parent[b])) {
+      hx::set_error("child visited after its parent");
+      return HX_ERR_BAD_TOPOLOGY;
+    }
+    seen_parent[v] = 1;
+    parent[a] = v;
+    parent[b] = v;
+    kids[2 * v] = a;
+    kids[2 * v + 1] = b;
+  }
+  const int root = postorder[3 * (I - 1)];
+  HX_REQUIRE(root == nn - 1, HX_ERR_BAD_TOPOLOGY, "root must be node 2T-2");
+  for (int v = 0; v < nn; ++v) {
+    if (v != root && parent[v] < 0) {
+      hx::set_error("node " + std::to_string(v) + " has no parent");
+      return HX_ERR_BAD_TOPOLOGY;
+    }
+  }
+  // heights (children first) and depths (parents first)
+  std::vector<int> height(nn, 0), depth(nn, 0);
+  for (int r = 0; r < I; ++r) {
+    int v = postorder[3 * r];
+    height[v] = 1 + std::max(height[kids[2 * v]], height[kids[2 * v + 1]]);
+  }
+  for (int r = I - 1; r >= 0; --r) {
+    int v = postorder[3 * r];
+    depth[kids[2 * v]] = depth[v] + 1;
+    depth[kids[2 * v + 1]] = depth[v] + 1;
+  }
+  const int H = height[root];
+  int D = 0;
+  for (int r = 0; r < I; ++r) D = std::max(D, depth[postorder[3 * r]] + 1);
+  std::vector<std::vector<int32_t>> post_lv(H), pre_lv(D);
+  for (int r = 0; r < I; ++r) {
+    int v = postorder[3 * r];
+    post_lv[height[v] - 1].push_back(v);
+    pre_lv[depth[v]].push_back(v);
+  }
+  auto* plan = new hx_plan_s();
+  plan->device = device;
+  plan->T = T;
+  plan->I = I;
+  plan->root = root;
+  plan->kids = kids;
+  std::vector<int32_t> host(2 * nn + 2 * I);
+  std::copy(kids.begin(), kids.end(), host.begin());
+  int pos = 2 * nn;
+  plan->post_off.push_back(0);
+  for (auto& lv : post_lv) {
+    for (int v : lv) host[pos++] = v;
+    plan->post_off.push_back(pos - 2 * nn);
+  }
+  plan->pre_off.push_back(0);
+  for (auto& lv : pre_lv) {
+    for (int v : lv) host[pos++] = v;
+    plan->pre_off.push_back(pos - 2 * nn - I);
+  }
+  int prev = 0;
+  hipError_t e = hipGetDevice(&prev);
+  if (e == hipSuccess) e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipMalloc(&plan->d_buf, host.size() * sizeof(int32_t));
+  if (e == hipSuccess)
+    e = hipMemcpy(plan->d_buf, host.data(), host.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+  hipSetDevice(prev);
+  if (e != hipSuccess) {
+    hx::set_error(std::string("plan upload: ") + hipGetErrorString(e));
+    if (plan->d_buf) hipFree(plan->d_buf);
+    delete plan;
+    return HX_ERR_HIP;
+  }
+  plan->d_kids = plan->d_buf;
+  plan->d_post = plan->d_buf + 2 * nn;
+  plan->d_pre = plan->d_buf + 2 * nn + I;
+  *out = plan;
+  return HX_OK;
+}
+
+int hx_plan_destroy(hx_plan_t plan) {
+  if (!plan) return HX_OK;
+  if (plan->d_buf) hipFree(plan->d_buf);
+  delete plan;
+  return HX_OK;
+}
+
+int hx_plan_info(hx_plan_t plan, int* post_levels, int* pre_levels, int* max_width) {
+  HX_REQUIRE(plan != nullptr, HX_ERR_INVALID_ARGUMENT, "null plan");
+  if (post_levels) *post_levels = (int)plan->post_off.size() - 1;
+  if (pre_levels) *pre_levels = (int)plan->pre_off.size() - 1;
+  if (max_width) {
+    int w = 0;
+    for (size_t i = 0; i + 1 < plan->post_off.size(); ++i)
+      w = std::max(w, plan->post_off[i + 1] - plan->post_off[i]);
+    for (size_t i = 0; i + 1 < plan->pre_off.size(); ++i)
+      w = std::max(w, plan->pre_off[i + 1] - plan->pre_off[i]);
+    *max_width = w;
+  }
+  return HX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,88 @@
+// Shared helpers for the hiptree HIP sources (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#include <string>
+#include <vector>
+
+#include "hiptree.h"
+
+namespace hx {
+
+// Last error detail for the calling host thread.
+void set_error(const std::string& msg);
+const std::string& get_error();
+
+#define HX_HIP_TRY(expr)                                                        \
+  do {                                                                          \
+    hipError_t _e = (expr);                                                     \
+    if (_e != hipSuccess) {                                                     \
+      ::hx::set_error(std::string(#expr " -> ") + hipGetErrorString(_e));      \
+      return HX_ERR_HIP;                                                        \
+    }                                                                           \
+  } while (0)
+
+#define HX_REQUIRE(cond, code, msg)                                             \
+  do {                                                                          \
+    if (!(cond)) {                                                              \
+      ::hx::set_error(msg);                                                     \
+      return (code);                                                            \
+    }                                                                           \
+  } while (0)
+
+constexpr int kBlock = 256;      // threads per block for the pruning kernels
+constexpr int kWave = 64;        // CDNA wavefront
+constexpr int kMaxCodes = 256;   // tip table rows held in LDS
+constexpr int kTargetBlocks = 2048;  // ~8 resident blocks per CU on 256 CUs
+
+inline int next_pow2(int x) {
+  int p = 1;
+  while (p < x) p <<= 1;
+  return p;
+}
+
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Pattern tiling shared by every pass of one call: each block owns a chunk of
+// consecutive pattern tiles for one (sample, node).
+struct Tiling {
+  int cp;               // lanes per pattern (C rounded up to a power of two)
+  int ppb;              // patterns per tile (= kBlock / cp)
+  int ntiles;
+  int nchunks;
+  int tiles_per_chunk;
+};
+
+inline Tiling make_tiling(int N, int C, int K) {
+  Tiling t;
+  t.cp = next_pow2(C < 1 ? 1 : C);
+  t.ppb = kBlock / t.cp;
+  t.ntiles = (N + t.ppb - 1) / t.ppb;
+  int want = (kTargetBlocks + K - 1) / K;
+  if (want < 8) want = 8;
+  int nch = want < t.ntiles ? want : t.ntiles;
+  if (nch < 1) nch = 1;
+  t.tiles_per_chunk = (t.ntiles + nch - 1) / nch;
+  t.nchunks = (t.ntiles + t.tiles_per_chunk - 1) / t.tiles_per_chunk;
+  return t;
+}
+
+}  // namespace hx
+
+// Opaque plan: one topology with its device-resident schedules.
+struct hx_plan_s {
+  int device = 0;
+  int T = 0;
+  int I = 0;
+  int root = 0;
+  int32_t* d_buf = nullptr;   // device: kids[2(2T-1)] | post_nodes[I] | pre_nodes[I]
+  const int32_t* d_kids = nullptr;
+  const int32_t* d_post = nullptr;
+  const int32_t* d_pre = nullptr;
+  std::vector<int> post_off;  // host: level offsets into d_post (size levels+1)
+  std::vector<int> pre_off;   // host: depth offsets into d_pre
+  std::vector<int32_t> kids;  // host copy of kids
+};

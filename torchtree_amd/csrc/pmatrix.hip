@@ -1,0 +1,268 @@
+// Transition matrices P = L diag(exp(lambda * t * r_c)) R from an FP64
+// eigensystem, and the chain rule back from d loglik / d P.
+//
+// Reference: SymmetricSubstitutionModel.p_t (substitution_model/abstract.py:57-79)
+// computes the same product with diag_embed and two batched matmuls, and gets
+// its gradient from autograd through torch.linalg.eigh (abstract.py:79).
+// Here the exponentials and products stay in FP64; the output is cast to the
+// pruning precision. The backward uses the Frechet derivative of exp in the
+// eigenbasis with divided differences of exp, which is exact at repeated
+// eigenvalues (where the eigh backward is undefined).
+#include "common.hpp"
+
+#include <math.h>
+
+namespace hx {
+
+// One thread per matrix element; S is small (<= 8), so each element
+// recomputes its S exponentials.
+template <typename O>
+__global__ void k_pmatrix_elem(int S, int K, int B, int C, int eb, const double* __restrict__ lam,
+                               const double* __restrict__ L, const double* __restrict__ R,
+                               const double* __restrict__ bls, const double* __restrict__ rates,
+                               int rb, O* __restrict__ out) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t SS = (size_t)S * S;
+  const size_t total = (size_t)K * B * C * SS;
+  if (idx >= total) return;
+  const int ij = (int)(idx % SS);
+  const size_t mat = idx / SS;
+  const int c = (int)(mat % C);
+  const size_t kb = mat / C;
+  const int b = (int)(kb % B), k = (int)(kb / B);
+  const int e = eb ? k : 0;
+  const double tau = bls[(size_t)k * B + b] * rates[(size_t)(rb ? k : 0) * C + c];
+  const int i = ij / S, j = ij % S;
+  const double* le = L + (size_t)e * SS;
+  const double* re = R + (size_t)e * SS;
+  const double* la = lam + (size_t)e * S;
+  double acc = 0.0;
+  for (int m = 0; m < S; ++m) acc = fma(le[i * S + m] * exp(la[m] * tau), re[m * S + j], acc);
+  out[idx] = (O)acc;
+}
+
+// One block per matrix for larger S: exponentials staged in LDS.
+template <typename O>
+__global__ void k_pmatrix_block(int S, int K, int B, int C, int eb, const double* __restrict__ lam,
+                                const double* __restrict__ L, const double* __restrict__ R,
+                                const double* __restrict__ bls, const double* __restrict__ rates,
+                                int rb, O* __restrict__ out) {
+  __shared__ double ex[64];
+  const size_t mat = blockIdx.x;
+  const int c = (int)(mat % C);
+  const size_t kb = mat / C;
+  const int b = (int)(kb % B), k = (int)(kb / B);
+  const int e = eb ? k : 0;
+  const size_t SS = (size_t)S * S;
+  const double tau = bls[(size_t)k * B + b] * rates[(size_t)(rb ? k : 0) * C + c];
+  if (threadIdx.x < S) ex[threadIdx.x] = exp(lam[(size_t)e * S + threadIdx.x] * tau);
+  __syncthreads();
+  const double* le = L + (size_t)e * SS;
+  const double* re = R + (size_t)e * SS;
+  for (int ij = threadIdx.x; ij < S * S; ij += blockDim.x) {
+    const int i = ij / S, j = ij % S;
+    double acc = 0.0;
+    for (int m = 0; m < S; ++m) acc = fma(le[i * S + m] * ex[m], re[m * S + j], acc);
+    out[mat * SS + ij] = (O)acc;
+  }
+}
+
+// (exp(a) - exp(b)) / (a - b) written as exp(b) * expm1(d) / d, d = a - b.
+__device__ __forceinline__ double exp_divdiff(double a, double b, double eb) {
+  const double d = a - b;
+  if (d == 0.0) return eb;
+  return eb * (expm1(d) / d);
+}
+
+// Per (k, b): for every category, Y = L^T G R^T, dtau = sum_m lam_m e^{lam_m tau} Y_mm and
+// xpart[k][b] += tau * (Y o F), F the divided differences of exp(lam * tau).
+__global__ void k_pgrad_branch(int S, int K, int B, int C, int eb, const double* __restrict__ lam,
+                               const double* __restrict__ L, const double* __restrict__ R,
+                               const double* __restrict__ bls, const double* __restrict__ rates,
+                               int rb, const double* __restrict__ dmats, double* __restrict__ dtau,
+                               double* __restrict__ xpart) {
+  extern __shared__ double sm[];
+  const int SS = S * S;
+  double* G = sm;          // [S][S], reused for Y
+  double* tmp = sm + SS;   // [S][S]
+  double* ex = sm + 2 * SS;  // [S]
+  double* dsum = ex + S;   // [1]
+  const int b = blockIdx.x % B, k = blockIdx.x / B;
+  const int e = eb ? k : 0;
+  const double* le = L + (size_t)e * SS;
+  const double* re = R + (size_t)e * SS;
+  const double* la = lam + (size_t)e * S;
+  double* X = xpart + ((size_t)k * B + b) * SS;
+  for (int ij = threadIdx.x; ij < SS; ij += blockDim.x) X[ij] = 0.0;
+  for (int c = 0; c < C; ++c) {
+    const double tau = bls[(size_t)k * B + b] * rates[(size_t)(rb ? k : 0) * C + c];
+    const double* g = dmats + (((size_t)k * B + b) * C + c) * SS;
+    for (int ij = threadIdx.x; ij < SS; ij += blockDim.x) G[ij] = g[ij];
+    if (threadIdx.x < S) ex[threadIdx.x] = exp(la[threadIdx.x] * tau);
+    if (threadIdx.x == 0) dsum[0] = 0.0;
+    __syncthreads();
+    // tmp = G R^T : tmp[i][n] = sum_j G[i][j] R[n][j]
+    for (int in = threadIdx.x; in < SS; in += blockDim.x) {
+      const int i = in / S, n = in % S;
+      double acc = 0.0;
+      for (int j = 0; j < S; ++j) acc = fma(G[i * S + j], re[n * S + j], acc);
+      tmp[in] = acc;
+    }
+    __syncthreads();
+    // Y = L^T tmp : Y[m][n] = sum_i L[i][m] tmp[i][n]  (into G)
+    for (int mn = threadIdx.x; mn < SS; mn += blockDim.x) {
+      const int m = mn / S, n = mn % S;
+      double acc = 0.0;
+      for (int i = 0; i < S; ++i) acc = fma(le[i * S + m], tmp[i * S + n], acc);
+      G[mn] = acc;
+    }
+    __syncthreads();
+    for (int mn = threadIdx.x; mn < SS; mn += blockDim.x) {
+      const int m = mn / S, n = mn % S;
+      const double F = exp_divdiff(la[m] * tau, la[n] * tau, ex[n]);
+      X[mn] += tau * G[mn] * F;
+      if (m == n) atomicAdd(dsum, la[m] * ex[m] * G[mn]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) dtau[((size_t)k * B + b) * C + c] = dsum[0];
+    __syncthreads();
+  }
+}
+
+__global__ void k_pgrad_bls(int K, int B, int C, const double* __restrict__ rates, int rb,
+                            const double* __restrict__ dtau, double* __restrict__ dbls) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)K * B) return;
+  const int k = (int)(idx / B);
+  double t = 0.0;
+  for (int c = 0; c < C; ++c) t += rates[(size_t)(rb ? k : 0) * C + c] * dtau[idx * C + c];
+  dbls[idx] = t;
+}
+
+__global__ void k_pgrad_rates(int K, int B, int C, const double* __restrict__ bls,
+                              const double* __restrict__ dtau, double* __restrict__ drates) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= K * C) return;
+  const int k = idx / C, c = idx % C;
+  double t = 0.0;
+  for (int b = 0; b < B; ++b) t += bls[(size_t)k * B + b] * dtau[((size_t)k * B + b) * C + c];
+  drates[idx] = t;
+}
+
+// Per eigensystem e: X = sum over its samples and branches of xpart; dQ = R^T X L^T.
+__global__ void k_pgrad_q(int S, int K, int B, int eb, const double* __restrict__ L,
+                          const double* __restrict__ R, const double* __restrict__ xpart,
+                          double* __restrict__ dq) {
+  extern __shared__ double sm[];
+  const int SS = S * S;
+  double* X = sm;
+  double* tmp = sm + SS;
+  const int e = blockIdx.x;
+  const int k0 = eb ? e : 0, k1 = eb ? e + 1 : K;
+  for (int mn = threadIdx.x; mn < SS; mn += blockDim.x) {
+    double t = 0.0;
+    for (int k = k0; k < k1; ++k)
+      for (int b = 0; b < B; ++b) t += xpart[((size_t)k * B + b) * SS + mn];
+    X[mn] = t;
+  }
+  __syncthreads();
+  const double* le = L + (size_t)e * SS;
+  const double* re = R + (size_t)e * SS;
+  // tmp = X L^T : tmp[m][j] = sum_n X[m][n] L[j][n]
+  for (int mj = threadIdx.x; mj < SS; mj += blockDim.x) {
+    const int m = mj / S, j = mj % S;
+    double acc = 0.0;
+    for (int n = 0; n < S; ++n) acc = fma(X[m * S + n], le[j * S + n], acc);
+    tmp[mj] = acc;
+  }
+  __syncthreads();
+  // dq = R^T tmp : dq[i][j] = sum_m R[m][i] tmp[m][j]
+  for (int ij = threadIdx.x; ij < SS; ij += blockDim.x) {
+    const int i = ij / S, j = ij % S;
+    double acc = 0.0;
+    for (int m = 0; m < S; ++m) acc = fma(re[m * S + i], tmp[m * S + j], acc);
+    dq[(size_t)e * SS + ij] = acc;
+  }
+}
+
+}  // namespace hx
+
+extern "C" {
+
+int hx_pmatrix(int S, int K, int B, int C, int eigen_batched, const double* lambda,
+               const double* left, const double* right, const double* bls, const double* rates,
+               int rates_batched, void* mats, int dtype, hx_stream_t stream) {
+  HX_REQUIRE(S >= 2 && S <= 64 && K >= 1 && B >= 1 && C >= 1, HX_ERR_INVALID_ARGUMENT,
+             "bad dimensions (S must be 2..64)");
+  HX_REQUIRE(lambda && left && right && bls && rates && mats, HX_ERR_INVALID_ARGUMENT,
+             "null pointer argument");
+  HX_REQUIRE(dtype == HX_F32 || dtype == HX_F64, HX_ERR_INVALID_ARGUMENT, "bad dtype");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const size_t nmat = (size_t)K * B * C;
+  if (S <= 8) {
+    const size_t total = nmat * S * S;
+    const unsigned grid = (unsigned)((total + 255) / 256);
+    if (dtype == HX_F64)
+      hx::k_pmatrix_elem<double><<<grid, 256, 0, st>>>(S, K, B, C, eigen_batched, lambda, left,
+                                                        right, bls, rates, rates_batched,
+                                                        static_cast<double*>(mats));
+    else
+      hx::k_pmatrix_elem<float><<<grid, 256, 0, st>>>(S, K, B, C, eigen_batched, lambda, left,
+                                                       right, bls, rates, rates_batched,
+                                                       static_cast<float*>(mats));
+  } else {
+    HX_REQUIRE(nmat < (1u << 31), HX_ERR_UNSUPPORTED, "too many matrices");
+    if (dtype == HX_F64)
+      hx::k_pmatrix_block<double><<<(unsigned)nmat, 256, 0, st>>>(S, K, B, C, eigen_batched, lambda,
+                                                                  left, right, bls, rates,
+                                                                  rates_batched,
+                                                                  static_cast<double*>(mats));
+    else
+      hx::k_pmatrix_block<float><<<(unsigned)nmat, 256, 0, st>>>(S, K, B, C, eigen_batched, lambda,
+                                                                 left, right, bls, rates,
+                                                                 rates_batched,
+                                                                 static_cast<float*>(mats));
+  }
+  HX_HIP_TRY(hipGetLastError());
+  return HX_OK;
+}
+
+size_t hx_pmatrix_grad_scratch_bytes(int S, int K, int B) {
+  // dtau [K][B][C<=64] + xpart [K][B][S][S]
+  return hx::align_up((size_t)K * B * 64 * 8, 256) + hx::align_up((size_t)K * B * S * S * 8, 256);
+}
+
+int hx_pmatrix_grad(int S, int K, int B, int C, int eigen_batched, const double* lambda,
+                    const double* left, const double* right, const double* bls, const double* rates,
+                    int rates_batched, const double* dmats, double* out_dbls, double* out_drates,
+                    double* out_dq, void* scratch, hx_stream_t stream) {
+  HX_REQUIRE(S >= 2 && S <= 64 && K >= 1 && B >= 1 && C >= 1 && C <= 64, HX_ERR_INVALID_ARGUMENT,
+             "bad dimensions");
+  HX_REQUIRE(lambda && left && right && bls && rates && dmats && scratch, HX_ERR_INVALID_ARGUMENT,
+             "null pointer argument");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  double* dtau = static_cast<double*>(scratch);
+  double* xpart = reinterpret_cast<double*>(static_cast<char*>(scratch) +
+                                            hx::align_up((size_t)K * B * 64 * 8, 256));
+  const size_t sm1 = (2 * (size_t)S * S + S + 1) * sizeof(double);
+  hx::k_pgrad_branch<<<(unsigned)(K * B), 256, sm1, st>>>(S, K, B, C, eigen_batched, lambda, left,
+                                                          right, bls, rates, rates_batched, dmats,
+                                                          dtau, xpart);
+  HX_HIP_TRY(hipGetLastError());
+  if (out_dbls)
+    hx::k_pgrad_bls<<<(unsigned)((K * (size_t)B + 255) / 256), 256, 0, st>>>(K, B, C, rates,
+                                                                              rates_batched, dtau,
+                                                                              out_dbls);
+  if (out_drates)
+    hx::k_pgrad_rates<<<(unsigned)((K * C + 255) / 256), 256, 0, st>>>(K, B, C, bls, dtau,
+                                                                       out_drates);
+  if (out_dq) {
+    const int E = eigen_batched ? K : 1;
+    const size_t sm2 = 2 * (size_t)S * S * sizeof(double);
+    hx::k_pgrad_q<<<E, 256, sm2, st>>>(S, K, B, eigen_batched, left, right, xpart, out_dq);
+  }
+  HX_HIP_TRY(hipGetLastError());
+  return HX_OK;
+}
+
+}  // extern "C"
