@@ -1,0 +1,161 @@
+"""Pins the oracle's P(t), Q builders and site models.
+
+Goldens from the reference tests: the R-derived GTR P(0.1) table and its
+batched form (test/test_substitution_model.py:53-136), the HKY tables
+(:139-213), JC69 closed form (:216-230), general symmetric == GTR (:31-50),
+MG94 rows (:298-310), Weibull rate tables (test/test_site_model.py:26-137).
+Independent checks: scipy.linalg.expm of the same Q, CTMC identities
+(rows sum to 1, P(0)=I, P(s)P(t)=P(s+t), detailed balance, pi P = pi) and
+scipy.special.gammaincinv for the gamma quantiles.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import models
+
+F64 = torch.float64
+RATES = torch.tensor([0.060602, 0.402732, 0.028230, 0.047910, 0.407249, 0.053277], dtype=F64)
+PI = torch.tensor([0.479367, 0.172572, 0.140933, 0.207128], dtype=F64)
+GTR_P01 = torch.tensor([
+    [0.93717830, 0.009506685, 0.047505899, 0.005809115],
+    [0.02640748, 0.894078744, 0.006448058, 0.073065722],
+    [0.16158572, 0.007895626, 0.820605951, 0.009912704],
+    [0.01344433, 0.060875872, 0.006744752, 0.918935042]], dtype=F64)
+HKY_P = torch.tensor([
+    [[0.93211187, 0.01511617, 0.03462891, 0.01814305],
+     [0.04198939, 0.89405292, 0.01234480, 0.05161289],
+     [0.11778615, 0.01511617, 0.84895463, 0.01814305],
+     [0.04198939, 0.04300210, 0.01234480, 0.90266370]],
+    [[0.9992649548, 0.0001581235, 0.0003871353, 0.0001897863],
+     [0.0004392323, 0.9988625812, 0.0001291335, 0.0005690531],
+     [0.0013167952, 0.0001581235, 0.9983352949, 0.0001897863],
+     [0.0004392323, 0.0004741156, 0.0001291335, 0.9989575186]]], dtype=F64)
+
+
+def test_gtr_table():
+    P = models.p_t_eigh(models.gtr_q(RATES, PI), PI, torch.tensor([[0.1]], dtype=F64))
+    assert torch.allclose(P.squeeze(), GTR_P01, rtol=1e-5)
+
+
+def test_gtr_batch_table():
+    rates = torch.stack((RATES, torch.tensor([1.0, 3.0, 1.0, 1.0, 3.0, 1.0], dtype=F64)))
+    pi = PI.expand(2, 4)
+    P = models.p_t_eigh(models.gtr_q(rates, pi), pi, torch.tensor([[[0.1]], [[0.001]]], dtype=F64))
+    assert torch.allclose(P[0, 0, 0], GTR_P01, rtol=1e-5)
+    assert torch.allclose(P[1, 0, 0], HKY_P[1], rtol=1e-5)
+
+
+def test_hky_table():
+    """HKY kappa=3 at t=0.1 and t=0.001 (test_substitution_model.py:139-175)."""
+    q = models.hky_q(torch.tensor([3.0], dtype=F64), PI)
+    P = models.p_t_eigh(q, PI, torch.tensor([[0.1], [0.001]], dtype=F64))
+    assert torch.allclose(P[0, 0], HKY_P[0], atol=1e-6)
+    assert torch.allclose(P[1, 0], HKY_P[1], atol=1e-6)
+
+
+@pytest.mark.parametrize("t", [0.001, 0.1, 1.7])
+def test_jc69_closed_form(t):
+    P = models.jc69_p_t(torch.tensor([t], dtype=F64))[0]
+    ii = 0.25 + 0.75 * np.exp(-4.0 / 3.0 * t)
+    ij = 0.25 - 0.25 * np.exp(-4.0 / 3.0 * t)
+    assert abs(float(P[0, 0]) - ii) < 1e-15 and abs(float(P[0, 1]) - ij) < 1e-15
+    # and equal to the eigen route with equal rates, uniform pi
+    Pe = models.p_t_eigh(models.gtr_q(torch.ones(6, dtype=F64), torch.full((4,), 0.25, dtype=F64)),
+                         torch.full((4,), 0.25, dtype=F64), torch.tensor([[t]], dtype=F64))[0, 0]
+    assert torch.allclose(P, Pe, atol=1e-14)
+
+
+def test_general_symmetric_equals_gtr():
+    qg = models.general_symmetric_q(RATES, torch.arange(6), PI)
+    assert torch.allclose(qg, models.gtr_q(RATES, PI), rtol=0, atol=0)
+
+
+def test_general_jc69_equals_jc69():
+    t = torch.tensor([[0.1], [0.5]], dtype=F64)
+    assert torch.allclose(models.general_jc69_p_t(t, 4), models.jc69_p_t(t), rtol=1e-15)
+
+
+def test_mg94_rows():
+    """test_substitution_model.py:298-310: alpha=beta=kappa=1, uniform pi."""
+    one = torch.tensor([1.0], dtype=F64)
+    pi = torch.full((61,), 1.0 / 61, dtype=F64)
+    q = models.mg94_q(one, one, one, pi)
+    assert torch.allclose(q[0, 1:], torch.full((60,), 1 / 61, dtype=F64))
+    assert torch.allclose(torch.diagonal(q), torch.full((61,), -60 / 61, dtype=F64))
+
+
+def test_mg94_flags_counts():
+    """Ordered one-step sense-to-sense mutations of the universal code: 61*9 - 23 = 526."""
+    ts, syn, non = models.mg94_masks()
+    tab, trip = models.UNIVERSAL, models.codon_triplets()
+    ordered = 0
+    for a in range(64):
+        if tab[a] == "*":
+            continue
+        for p in range(3):
+            for b in "ACGT":
+                t = trip[a][:p] + b + trip[a][p + 1:]
+                if b != trip[a][p] and tab[trip.index(t)] != "*":
+                    ordered += 1
+    assert ordered == 526
+    assert int((syn + non).sum()) * 2 == ordered
+    assert torch.all((ts == 0) | ((syn + non) == 1))
+    assert torch.all(syn * non == 0)
+
+
+@pytest.mark.parametrize("seed,S", [(0, 4), (1, 20), (2, 61)])
+def test_expm_and_identities(seed, S):
+    from scipy.linalg import expm
+
+    rng = np.random.default_rng(seed)
+    pi = torch.tensor(rng.dirichlet(np.ones(S) * 3), dtype=F64)
+    ex = torch.tensor(rng.gamma(1.0, 1.0, S * (S - 1) // 2), dtype=F64)
+    q = models.q_from_exchangeabilities(ex, pi)
+    qn = models.normalize_q(q, pi)
+    assert abs(float(-(torch.diagonal(qn) * pi).sum()) - 1.0) < 1e-14
+    ts = torch.tensor([[0.0, 0.05, 0.3, 2.0]], dtype=F64)
+    P = models.p_t_eigh(q, pi, ts)[0]
+    for i, t in enumerate(ts[0]):
+        ref = expm(qn.numpy() * float(t))
+        assert np.allclose(P[i].numpy(), ref, atol=1e-12)
+    assert torch.allclose(P.sum(-1), torch.ones(4, S, dtype=F64), atol=1e-12)
+    assert torch.allclose(P[0], torch.eye(S, dtype=F64), atol=1e-12)
+    Pst = models.p_t_eigh(q, pi, torch.tensor([[0.35]], dtype=F64))[0, 0]
+    assert torch.allclose(P[1] @ P[2], Pst, atol=1e-12)
+    db = pi[:, None] * P[2]
+    assert torch.allclose(db, db.t(), atol=1e-13)
+    assert torch.allclose(pi @ P[3], pi, atol=1e-12)
+
+
+def test_weibull_tables():
+    r1, p1 = models.weibull_rates_probs(torch.tensor([[1.0], [0.1]], dtype=F64), 4)
+    exp = torch.tensor([[0.1457844, 0.5131316, 1.0708310, 2.2702530],
+                        [4.766392e-12, 1.391131e-06, 2.179165e-03, 3.997819]], dtype=F64)
+    assert torch.allclose(r1, exp, rtol=1e-6)
+    assert torch.allclose((r1 * p1).sum(-1), torch.ones(2, dtype=F64), rtol=1e-14)
+    r2, p2 = models.weibull_rates_probs(torch.tensor([1.0], dtype=F64), 3,
+                                        invariant=torch.tensor([0.0], dtype=F64))
+    assert torch.allclose(r2, torch.tensor([0.0, 0.20506860315799713, 0.7796264290809631,
+                                            2.0153050422668457], dtype=F64), rtol=1e-6)
+    r3, _ = models.weibull_rates_probs(torch.tensor([1.0], dtype=F64), 4,
+                                       mu=torch.tensor([2.0], dtype=F64))
+    assert torch.allclose(r3, exp[0] * 2, rtol=1e-6)
+
+
+def test_invariant():
+    r, p = models.invariant_rates_probs(torch.tensor([0.2], dtype=F64),
+                                        mu=torch.tensor([2.0], dtype=F64))
+    assert abs(float(r[1]) - 2 / 0.8) < 1e-15
+    assert torch.allclose(p, torch.tensor([0.2, 0.8], dtype=F64))
+
+
+def test_gamma_quantiles():
+    from scipy.special import gammaincinv
+    from scipy.stats import gamma as gdist
+
+    r, p = models.gamma_rates_probs(0.5, 4)
+    q = (2 * np.arange(4) + 1) / 8
+    raw = gdist.ppf(q, 0.5, scale=1 / 0.5)
+    assert np.allclose(r.numpy(), raw / raw.mean(), rtol=1e-12)
+    assert abs(float((r * p).sum()) - 1.0) < 1e-14

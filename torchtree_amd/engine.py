@@ -62,9 +62,11 @@ class _TreeLikFn(torch.autograd.Function):
         dp = ctx.dprops * g.view(K, 1)
         if props.dim() == 1 or props.shape[0] != K or K == 1:
             dp = dp.sum(0).reshape(props.shape)
-        return (dbls.to(bls.dtype), drates.reshape(rates.shape).to(rates.dtype),
-                dp.to(props.dtype), dq.reshape(q_norm.shape).to(q_norm.dtype),
-                df.to(freqs.dtype), None)
+        return (dbls.to(bls.device, bls.dtype),
+                drates.reshape(rates.shape).to(rates.device, rates.dtype),
+                dp.to(props.device, props.dtype),
+                dq.reshape(q_norm.shape).to(q_norm.device, q_norm.dtype),
+                df.to(freqs.device, freqs.dtype), None)
 
 
 class TreeJob:
