@@ -33,7 +33,7 @@ import math
 import numpy as np
 import torch
 
-from . import _lib
+from . import _lib, timing
 from ._lib import HX_F32, HX_F64, HX_TIPS_CODES, HX_TIPS_DENSE, HipTreeError
 
 
@@ -206,7 +206,8 @@ class Pruning:
         out = torch.empty(K, dtype=torch.float64, device=dev)
         dfreqs = torch.empty(K, S, dtype=torch.float64, device=dev) if want_root_grads else None
         dprops = torch.empty(K, C, dtype=torch.float64, device=dev) if want_root_grads else None
-        rc = lib.hx_loglik(
+        with timing.phase("postorder", self.plan.post_levels + 1):
+            rc = lib.hx_loglik(
             self.plan.handle, S, self.N, C, K, code, self.tips.kind,
             _lib.ptr(tips_p), _lib.ptr(table_p), self.tips.ncodes,
             _lib.ptr(self.weights), _lib.ptr(mats),
@@ -224,7 +225,8 @@ class Pruning:
         K, B, C, S, code, tips_p, table_p = self._common(mats, freqs, props, dtype)
         dev = mats.device
         dmats = torch.empty(K, B, C, S, S, dtype=torch.float64, device=dev)
-        rc = lib.hx_loglik_grad_mats(
+        with timing.phase("preorder", self.plan.pre_levels + 1):
+            rc = lib.hx_loglik_grad_mats(
             self.plan.handle, S, self.N, C, K, code, self.tips.kind,
             _lib.ptr(tips_p), _lib.ptr(table_p), self.tips.ncodes,
             _lib.ptr(self.weights), _lib.ptr(mats),

@@ -24,7 +24,7 @@ import itertools
 
 import torch
 
-from . import _lib
+from . import _lib, timing
 from ._lib import HX_F32, HX_F64, HipTreeError
 
 F64 = torch.float64
@@ -195,7 +195,8 @@ def pmatrix(eig: Eigen, bls: torch.Tensor, rates: torch.Tensor, dtype) -> torch.
     rates = rates.to(F64).contiguous()
     out = torch.empty(K, B, C, eig.S, eig.S, dtype=dtype, device=bls.device)
     code = HX_F64 if dtype == F64 else HX_F32
-    rc = lib.hx_pmatrix(eig.S, K, B, C, int(eig.batched), _lib.ptr(eig.lam), _lib.ptr(eig.left),
+    with timing.phase("pmatrix"):
+        rc = lib.hx_pmatrix(eig.S, K, B, C, int(eig.batched), _lib.ptr(eig.lam), _lib.ptr(eig.left),
                         _lib.ptr(eig.right), _lib.ptr(bls), _lib.ptr(rates),
                         int(rates.shape[0] == K and K > 1), _lib.ptr(out), code,
                         _lib.stream_handle(bls.device))
@@ -217,7 +218,8 @@ def pmatrix_grad(eig: Eigen, bls, rates, dmats):
     dbls = torch.empty(K, B, dtype=F64, device=dev)
     drates = torch.empty(K, C, dtype=F64, device=dev)
     dq = torch.empty(eig.E, eig.S, eig.S, dtype=F64, device=dev)
-    rc = lib.hx_pmatrix_grad(eig.S, K, B, C, int(eig.batched), _lib.ptr(eig.lam),
+    with timing.phase("pmatrix_grad", 4):
+        rc = lib.hx_pmatrix_grad(eig.S, K, B, C, int(eig.batched), _lib.ptr(eig.lam),
                              _lib.ptr(eig.left), _lib.ptr(eig.right), _lib.ptr(bls),
                              _lib.ptr(rates), int(rates.shape[0] == K and K > 1), _lib.ptr(dmats),
                              _lib.ptr(dbls), _lib.ptr(drates), _lib.ptr(dq), _lib.ptr(scratch),
