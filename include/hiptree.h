@@ -124,8 +124,8 @@ int hx_pmatrix(int S, int K, int B, int C, int eigen_batched,
  *                         that share an eigensystem)
  * using P = exp(Q tau) and the divided differences of exp on the spectrum
  * (exact at repeated eigenvalues). `scratch` needs
- * hx_pmatrix_grad_scratch_bytes(S, K, B) bytes of device memory. */
-size_t hx_pmatrix_grad_scratch_bytes(int S, int K, int B);
+ * hx_pmatrix_grad_scratch_bytes(S, K, B, C) bytes of device memory. */
+size_t hx_pmatrix_grad_scratch_bytes(int S, int K, int B, int C);
 int hx_pmatrix_grad(int S, int K, int B, int C, int eigen_batched,
                     const double* lambda, const double* left, const double* right,
                     const double* bls, const double* rates, int rates_batched,

@@ -45,7 +45,7 @@ _SIGNATURES = {
                                      _vp, _c_size, _dp, _vp]),
     "hx_pmatrix": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int,
                             _dp, _dp, _dp, _dp, _dp, _c_int, _vp, _c_int, _vp]),
-    "hx_pmatrix_grad_scratch_bytes": (_c_size, [_c_int, _c_int, _c_int]),
+    "hx_pmatrix_grad_scratch_bytes": (_c_size, [_c_int, _c_int, _c_int, _c_int]),
     "hx_pmatrix_grad": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int,
                                  _dp, _dp, _dp, _dp, _dp, _c_int,
                                  _dp, _dp, _dp, _dp, _vp, _vp]),

@@ -74,59 +74,115 @@ __device__ __forceinline__ double exp_divdiff(double a, double b, double eb) {
   return eb * (expm1(d) / d);
 }
 
-// Per (k, b): for every category, Y = L^T G R^T, dtau = sum_m lam_m e^{lam_m tau} Y_mm and
-// xpart[k][b] += tau * (Y o F), F the divided differences of exp(lam * tau).
-__global__ void k_pgrad_branch(int S, int K, int B, int C, int eb, const double* __restrict__ lam,
-                               const double* __restrict__ L, const double* __restrict__ R,
-                               const double* __restrict__ bls, const double* __restrict__ rates,
-                               int rb, const double* __restrict__ dmats, double* __restrict__ dtau,
-                               double* __restrict__ xpart) {
+// Small S: one thread per (k, b, c). Y = L^T G R^T, dtau = sum_m lam_m e^{lam_m tau} Y_mm,
+// xpart[k][b][c] = tau * (Y o F) with F the divided differences of exp(lam * tau).
+template <int S>
+__global__ void k_pgrad_small(int K, int B, int C, int eb, const double* __restrict__ lam,
+                              const double* __restrict__ L, const double* __restrict__ R,
+                              const double* __restrict__ bls, const double* __restrict__ rates,
+                              int rb, const double* __restrict__ dmats, double* __restrict__ dtau,
+                              double* __restrict__ xpart) {
+  constexpr int SS = S * S;
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)K * B * C) return;
+  const int c = (int)(idx % C);
+  const size_t kb = idx / C;
+  const int b = (int)(kb % B), k = (int)(kb / B);
+  const int e = eb ? k : 0;
+  const double tau = bls[(size_t)k * B + b] * rates[(size_t)(rb ? k : 0) * C + c];
+  const double* le = L + (size_t)e * SS;
+  const double* re = R + (size_t)e * SS;
+  const double* la = lam + (size_t)e * S;
+  double G[SS], tmp[SS];
+#pragma unroll
+  for (int q = 0; q < SS; ++q) G[q] = dmats[idx * SS + q];
+#pragma unroll
+  for (int i = 0; i < S; ++i)
+#pragma unroll
+    for (int n = 0; n < S; ++n) {
+      double acc = 0.0;
+#pragma unroll
+      for (int j = 0; j < S; ++j) acc = fma(G[i * S + j], re[n * S + j], acc);
+      tmp[i * S + n] = acc;
+    }
+  double a[S], ex[S];
+#pragma unroll
+  for (int m = 0; m < S; ++m) {
+    a[m] = la[m] * tau;
+    ex[m] = exp(a[m]);
+  }
+  double dt = 0.0;
+#pragma unroll
+  for (int m = 0; m < S; ++m)
+#pragma unroll
+    for (int n = 0; n < S; ++n) {
+      double y = 0.0;
+#pragma unroll
+      for (int i = 0; i < S; ++i) y = fma(le[i * S + m], tmp[i * S + n], y);
+      if (m == n) dt = fma(la[m] * ex[m], y, dt);
+      xpart[idx * SS + m * S + n] = tau * y * exp_divdiff(a[m], a[n], ex[n]);
+    }
+  dtau[idx] = dt;
+}
+
+// Any S <= 64: one block per (k, b, c), staged through LDS.
+__global__ void k_pgrad_block(int S, int K, int B, int C, int eb, const double* __restrict__ lam,
+                              const double* __restrict__ L, const double* __restrict__ R,
+                              const double* __restrict__ bls, const double* __restrict__ rates,
+                              int rb, const double* __restrict__ dmats, double* __restrict__ dtau,
+                              double* __restrict__ xpart) {
   extern __shared__ double sm[];
   const int SS = S * S;
-  double* G = sm;          // [S][S], reused for Y
-  double* tmp = sm + SS;   // [S][S]
+  double* G = sm;            // [S][S], reused for Y
+  double* tmp = sm + SS;     // [S][S]
   double* ex = sm + 2 * SS;  // [S]
-  double* dsum = ex + S;   // [1]
-  const int b = blockIdx.x % B, k = blockIdx.x / B;
+  const size_t idx = blockIdx.x;
+  const int c = (int)(idx % C);
+  const size_t kb = idx / C;
+  const int b = (int)(kb % B), k = (int)(kb / B);
   const int e = eb ? k : 0;
   const double* le = L + (size_t)e * SS;
   const double* re = R + (size_t)e * SS;
   const double* la = lam + (size_t)e * S;
-  double* X = xpart + ((size_t)k * B + b) * SS;
-  for (int ij = threadIdx.x; ij < SS; ij += blockDim.x) X[ij] = 0.0;
-  for (int c = 0; c < C; ++c) {
-    const double tau = bls[(size_t)k * B + b] * rates[(size_t)(rb ? k : 0) * C + c];
-    const double* g = dmats + (((size_t)k * B + b) * C + c) * SS;
-    for (int ij = threadIdx.x; ij < SS; ij += blockDim.x) G[ij] = g[ij];
-    if (threadIdx.x < S) ex[threadIdx.x] = exp(la[threadIdx.x] * tau);
-    if (threadIdx.x == 0) dsum[0] = 0.0;
-    __syncthreads();
-    // tmp = G R^T : tmp[i][n] = sum_j G[i][j] R[n][j]
-    for (int in = threadIdx.x; in < SS; in += blockDim.x) {
-      const int i = in / S, n = in % S;
-      double acc = 0.0;
-      for (int j = 0; j < S; ++j) acc = fma(G[i * S + j], re[n * S + j], acc);
-      tmp[in] = acc;
-    }
-    __syncthreads();
-    // Y = L^T tmp : Y[m][n] = sum_i L[i][m] tmp[i][n]  (into G)
-    for (int mn = threadIdx.x; mn < SS; mn += blockDim.x) {
-      const int m = mn / S, n = mn % S;
-      double acc = 0.0;
-      for (int i = 0; i < S; ++i) acc = fma(le[i * S + m], tmp[i * S + n], acc);
-      G[mn] = acc;
-    }
-    __syncthreads();
-    for (int mn = threadIdx.x; mn < SS; mn += blockDim.x) {
-      const int m = mn / S, n = mn % S;
-      const double F = exp_divdiff(la[m] * tau, la[n] * tau, ex[n]);
-      X[mn] += tau * G[mn] * F;
-      if (m == n) atomicAdd(dsum, la[m] * ex[m] * G[mn]);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) dtau[((size_t)k * B + b) * C + c] = dsum[0];
+  const double tau = bls[(size_t)k * B + b] * rates[(size_t)(rb ? k : 0) * C + c];
+  const double* g = dmats + idx * SS;
+  for (int ij = threadIdx.x; ij < SS; ij += blockDim.x) G[ij] = g[ij];
+  if (threadIdx.x < S) ex[threadIdx.x] = exp(la[threadIdx.x] * tau);
+  __syncthreads();
+  for (int in = threadIdx.x; in < SS; in += blockDim.x) {
+    const int i = in / S, n = in % S;
+    double acc = 0.0;
+    for (int j = 0; j < S; ++j) acc = fma(G[i * S + j], re[n * S + j], acc);
+    tmp[in] = acc;
+  }
+  __syncthreads();
+  for (int mn = threadIdx.x; mn < SS; mn += blockDim.x) {
+    const int m = mn / S, n = mn % S;
+    double acc = 0.0;
+    for (int i = 0; i < S; ++i) acc = fma(le[i * S + m], tmp[i * S + n], acc);
+    G[mn] = acc;
+  }
+  __syncthreads();
+  double* X = xpart + idx * SS;
+  for (int mn = threadIdx.x; mn < SS; mn += blockDim.x) {
+    const int m = mn / S, n = mn % S;
+    X[mn] = tau * G[mn] * exp_divdiff(la[m] * tau, la[n] * tau, ex[n]);
+  }
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int m = 0; m < S; ++m) t = fma(la[m] * ex[m], G[m * S + m], t);
+    dtau[idx] = t;
+  }
+}
+
+__device__ __forceinline__ double block_reduce_sum(double v, double* red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
     __syncthreads();
   }
+  return red[0];
 }
 
 __global__ void k_pgrad_bls(int K, int B, int C, const double* __restrict__ rates, int rb,
@@ -139,36 +195,44 @@ __global__ void k_pgrad_bls(int K, int B, int C, const double* __restrict__ rate
   dbls[idx] = t;
 }
 
+// Block per (k, c): sum over branches of bls * dtau.
 __global__ void k_pgrad_rates(int K, int B, int C, const double* __restrict__ bls,
                               const double* __restrict__ dtau, double* __restrict__ drates) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= K * C) return;
-  const int k = idx / C, c = idx % C;
+  __shared__ double red[256];
+  const int k = blockIdx.x / C, c = blockIdx.x % C;
   double t = 0.0;
-  for (int b = 0; b < B; ++b) t += bls[(size_t)k * B + b] * dtau[((size_t)k * B + b) * C + c];
-  drates[idx] = t;
+  for (int b = threadIdx.x; b < B; b += blockDim.x)
+    t += bls[(size_t)k * B + b] * dtau[((size_t)k * B + b) * C + c];
+  t = block_reduce_sum(t, red);
+  if (threadIdx.x == 0) drates[blockIdx.x] = t;
 }
 
-// Per eigensystem e: X = sum over its samples and branches of xpart; dQ = R^T X L^T.
-__global__ void k_pgrad_q(int S, int K, int B, int eb, const double* __restrict__ L,
-                          const double* __restrict__ R, const double* __restrict__ xpart,
-                          double* __restrict__ dq) {
+// Block per (e, mn): X[e][mn] = sum over the samples of e, branches and categories of xpart.
+__global__ void k_pgrad_xsum(int SS, int K, int B, int C, int eb, const double* __restrict__ xpart,
+                             double* __restrict__ X) {
+  __shared__ double red[256];
+  const int e = blockIdx.x / SS, mn = blockIdx.x % SS;
+  const size_t per = (size_t)B * C;
+  const size_t r0 = eb ? (size_t)e * per : 0;
+  const size_t r1 = eb ? r0 + per : (size_t)K * per;
+  double t = 0.0;
+  for (size_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) t += xpart[r * SS + mn];
+  t = block_reduce_sum(t, red);
+  if (threadIdx.x == 0) X[blockIdx.x] = t;
+}
+
+// Per eigensystem e: dQ = R^T X L^T.
+__global__ void k_pgrad_q(int S, const double* __restrict__ L, const double* __restrict__ R,
+                          const double* __restrict__ Xall, double* __restrict__ dq) {
   extern __shared__ double sm[];
   const int SS = S * S;
   double* X = sm;
   double* tmp = sm + SS;
   const int e = blockIdx.x;
-  const int k0 = eb ? e : 0, k1 = eb ? e + 1 : K;
-  for (int mn = threadIdx.x; mn < SS; mn += blockDim.x) {
-    double t = 0.0;
-    for (int k = k0; k < k1; ++k)
-      for (int b = 0; b < B; ++b) t += xpart[((size_t)k * B + b) * SS + mn];
-    X[mn] = t;
-  }
+  for (int mn = threadIdx.x; mn < SS; mn += blockDim.x) X[mn] = Xall[(size_t)e * SS + mn];
   __syncthreads();
   const double* le = L + (size_t)e * SS;
   const double* re = R + (size_t)e * SS;
-  // tmp = X L^T : tmp[m][j] = sum_n X[m][n] L[j][n]
   for (int mj = threadIdx.x; mj < SS; mj += blockDim.x) {
     const int m = mj / S, j = mj % S;
     double acc = 0.0;
@@ -176,7 +240,6 @@ __global__ void k_pgrad_q(int S, int K, int B, int eb, const double* __restrict_
     tmp[mj] = acc;
   }
   __syncthreads();
-  // dq = R^T tmp : dq[i][j] = sum_m R[m][i] tmp[m][j]
   for (int ij = threadIdx.x; ij < SS; ij += blockDim.x) {
     const int i = ij / S, j = ij % S;
     double acc = 0.0;
@@ -227,39 +290,62 @@ int hx_pmatrix(int S, int K, int B, int C, int eigen_batched, const double* lamb
   return HX_OK;
 }
 
-size_t hx_pmatrix_grad_scratch_bytes(int S, int K, int B) {
-  // dtau [K][B][C<=64] + xpart [K][B][S][S]
-  return hx::align_up((size_t)K * B * 64 * 8, 256) + hx::align_up((size_t)K * B * S * S * 8, 256);
+static size_t pgrad_layout(int S, int K, int B, int C, size_t* o_xpart, size_t* o_x) {
+  const size_t n = (size_t)K * B * C;
+  size_t off = hx::align_up(n * 8, 256);
+  *o_xpart = off;
+  off += hx::align_up(n * (size_t)S * S * 8, 256);
+  *o_x = off;
+  off += hx::align_up((size_t)K * S * S * 8, 256);
+  return off;
+}
+
+size_t hx_pmatrix_grad_scratch_bytes(int S, int K, int B, int C) {
+  size_t a, b;
+  return pgrad_layout(S, K, B, C, &a, &b);
 }
 
 int hx_pmatrix_grad(int S, int K, int B, int C, int eigen_batched, const double* lambda,
                     const double* left, const double* right, const double* bls, const double* rates,
                     int rates_batched, const double* dmats, double* out_dbls, double* out_drates,
                     double* out_dq, void* scratch, hx_stream_t stream) {
-  HX_REQUIRE(S >= 2 && S <= 64 && K >= 1 && B >= 1 && C >= 1 && C <= 64, HX_ERR_INVALID_ARGUMENT,
+  HX_REQUIRE(S >= 2 && S <= 64 && K >= 1 && B >= 1 && C >= 1, HX_ERR_INVALID_ARGUMENT,
              "bad dimensions");
   HX_REQUIRE(lambda && left && right && bls && rates && dmats && scratch, HX_ERR_INVALID_ARGUMENT,
              "null pointer argument");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  double* dtau = static_cast<double*>(scratch);
-  double* xpart = reinterpret_cast<double*>(static_cast<char*>(scratch) +
-                                            hx::align_up((size_t)K * B * 64 * 8, 256));
-  const size_t sm1 = (2 * (size_t)S * S + S + 1) * sizeof(double);
-  hx::k_pgrad_branch<<<(unsigned)(K * B), 256, sm1, st>>>(S, K, B, C, eigen_batched, lambda, left,
-                                                          right, bls, rates, rates_batched, dmats,
-                                                          dtau, xpart);
+  size_t o_xpart, o_x;
+  pgrad_layout(S, K, B, C, &o_xpart, &o_x);
+  char* base = static_cast<char*>(scratch);
+  double* dtau = reinterpret_cast<double*>(base);
+  double* xpart = reinterpret_cast<double*>(base + o_xpart);
+  double* X = reinterpret_cast<double*>(base + o_x);
+  const size_t n = (size_t)K * B * C;
+  switch (S) {
+    case 4:
+      hx::k_pgrad_small<4><<<(unsigned)((n + 127) / 128), 128, 0, st>>>(
+          K, B, C, eigen_batched, lambda, left, right, bls, rates, rates_batched, dmats, dtau, xpart);
+      break;
+    default: {
+      HX_REQUIRE(n < (1u << 31), HX_ERR_UNSUPPORTED, "too many matrices");
+      const size_t sm1 = (2 * (size_t)S * S + S + 1) * sizeof(double);
+      hx::k_pgrad_block<<<(unsigned)n, 256, sm1, st>>>(S, K, B, C, eigen_batched, lambda, left,
+                                                       right, bls, rates, rates_batched, dmats,
+                                                       dtau, xpart);
+    }
+  }
   HX_HIP_TRY(hipGetLastError());
   if (out_dbls)
     hx::k_pgrad_bls<<<(unsigned)((K * (size_t)B + 255) / 256), 256, 0, st>>>(K, B, C, rates,
                                                                               rates_batched, dtau,
                                                                               out_dbls);
   if (out_drates)
-    hx::k_pgrad_rates<<<(unsigned)((K * C + 255) / 256), 256, 0, st>>>(K, B, C, bls, dtau,
-                                                                       out_drates);
+    hx::k_pgrad_rates<<<(unsigned)(K * C), 256, 0, st>>>(K, B, C, bls, dtau, out_drates);
   if (out_dq) {
     const int E = eigen_batched ? K : 1;
+    hx::k_pgrad_xsum<<<(unsigned)(E * S * S), 256, 0, st>>>(S * S, K, B, C, eigen_batched, xpart, X);
     const size_t sm2 = 2 * (size_t)S * S * sizeof(double);
-    hx::k_pgrad_q<<<E, 256, sm2, st>>>(S, K, B, eigen_batched, left, right, xpart, out_dq);
+    hx::k_pgrad_q<<<E, 256, sm2, st>>>(S, left, right, X, out_dq);
   }
   HX_HIP_TRY(hipGetLastError());
   return HX_OK;

@@ -213,7 +213,7 @@ def pmatrix_grad(eig: Eigen, bls, rates, dmats):
     bls = bls.to(F64).contiguous()
     rates = rates.to(F64).contiguous()
     dmats = dmats.to(F64).contiguous()
-    scratch = torch.empty(int(lib.hx_pmatrix_grad_scratch_bytes(eig.S, K, B)), dtype=torch.uint8,
+    scratch = torch.empty(int(lib.hx_pmatrix_grad_scratch_bytes(eig.S, K, B, C)), dtype=torch.uint8,
                           device=dev)
     dbls = torch.empty(K, B, dtype=F64, device=dev)
     drates = torch.empty(K, C, dtype=F64, device=dev)
