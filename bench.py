@@ -206,13 +206,13 @@ def main():
 
 
 def _nchunks(N, C, K):
+    """Chunk count of the S=4 kernels (mirrors make_tiling in csrc/common.hpp)."""
     cp = 1
     while cp < C:
         cp *= 2
     ppb = 256 // cp
     ntiles = (N + ppb - 1) // ppb
-    want = max(8, (2048 + K - 1) // K)
-    nch = max(1, min(want, ntiles))
+    nch = max(1, min(ntiles, 128))
     tpc = (ntiles + nch - 1) // nch
     return (ntiles + tpc - 1) // tpc
 

@@ -1,0 +1,160 @@
+"""The JSON TreeLikelihoodModel on the GPU against the reference goldens.
+
+Object graphs follow test/test_tree_likelihood.py:73-342: the tiny JC69
+unrooted model (-83.329016), the fluA JC69 + Weibull(0.1)x4 + strict clock
+model on a reparameterised time tree (-4618.2062529058, single and a batch
+of 3), and batch == single (exact equality, as the reference asserts).
+"""
+import pytest
+import torch
+
+from oracle import data as odata
+
+pytestmark = pytest.mark.gpu
+
+
+def _taxa(path, dated=False):
+    out = []
+    for t, _ in odata.read_fasta(path):
+        d = {"id": t, "type": "Taxon"}
+        if dated:
+            d["attributes"] = {"date": float(t.split("_")[-1])}
+        out.append(d)
+    return {"id": "taxa", "type": "Taxa", "taxa": out}
+
+
+@pytest.mark.parametrize("precision,tol", [("float64", 1e-9), ("float32", 1e-6)])
+def test_tiny_json(gpu, tiny_newick, tiny_fasta, precision, tol):
+    print(f"case: tiny JSON {precision}", flush=True)
+    from torchtree_amd.treelikelihood_model import TreeLikelihoodModel
+
+    data = {
+        "id": "like", "type": "TreeLikelihoodModel", "precision": precision,
+        "tree_model": {"id": "tree", "type": "UnRootedTreeModel", "newick": tiny_newick,
+                       "branch_lengths": {"id": "branches", "type": "torchtree.Parameter",
+                                          "tensor": [0.0], "dtype": "torch.float64"},
+                       "keep_branch_lengths": True, "taxa": _taxa(tiny_fasta)},
+        "site_model": {"id": "sm", "type": "ConstantSiteModel"},
+        "substitution_model": {"id": "m", "type": "JC69"},
+        "site_pattern": {"id": "sp", "type": "SitePattern",
+                         "alignment": {"id": "alignment", "type": "Alignment",
+                                       "datatype": "nucleotide", "file": tiny_fasta, "taxa": "taxa"}},
+    }
+    like = TreeLikelihoodModel.from_json(data, {})
+    out = like()
+    assert out.shape == (1,)
+    assert abs(float(out) - (-83.329016)) < 83.33 * max(tol, 1e-7) + 1e-6
+
+
+def _flu(flu_newick, flu_fasta, batch):
+    ratios = {"id": "ratios", "type": "Parameter", "tensor": [0.5] * 67, "dtype": "torch.float64"}
+    root = {"id": "root_height", "type": "Parameter", "tensor": [20.0], "dtype": "torch.float64"}
+    return {
+        "id": "like", "type": "torchtree.tree_likelihood.TreeLikelihoodModel", "precision": "float64",
+        "tree_model": {"id": "tree_model", "type": "ReparameterizedTimeTreeModel", "newick": flu_newick,
+                       "ratios": ratios, "root_height": root, "keep_branch_lengths": True,
+                       "taxa": _taxa(flu_fasta, dated=True)},
+        "site_model": {"id": "site_model", "type": "WeibullSiteModel", "categories": 4,
+                       "shape": {"id": "shape", "type": "Parameter", "tensor": [[0.1]] * batch,
+                                 "dtype": "torch.float64"}},
+        "substitution_model": {"id": "jc", "type": "JC69"},
+        "site_pattern": {"id": "sp", "type": "torchtree.evolution.site_pattern.SitePattern",
+                         "alignment": {"id": "alignment", "type": "torchtree.evolution.alignment.Alignment",
+                                       "datatype": "nucleotide", "file": flu_fasta, "taxa": "taxa"}},
+        "branch_model": {"id": "clock", "type": "StrictClockModel", "tree_model": "tree_model",
+                         "rate": {"id": "rate", "type": "Parameter", "tensor": [[0.001]] * batch,
+                                  "dtype": "torch.float64"}},
+    }
+
+
+def test_flu_json_golden_and_batch(gpu, flu_newick, flu_fasta):
+    print("case: fluA JSON reparameterised time tree", flush=True)
+    from torchtree_amd.treelikelihood_model import TreeLikelihoodModel
+
+    like = TreeLikelihoodModel.from_json(_flu(flu_newick, flu_fasta, 1), {})
+    single = like()
+    assert single.shape == (1, 1)
+    assert abs(float(single) - (-4618.2062529058)) < 4618.2 * 1e-5
+    like3 = TreeLikelihoodModel.from_json(_flu(flu_newick, flu_fasta, 3), {})
+    b3 = like3()
+    assert b3.shape == (3, 1)
+    assert torch.all(b3 == single)
+
+
+def test_batch_equals_single_exactly(gpu):
+    """test_tree_likelihood.py:172-265: time tree heights, Weibull, strict clock."""
+    print("case: batch == single", flush=True)
+    from torchtree_amd.treelikelihood_model import TreeLikelihoodModel
+
+    def graph(heights, shape, rate):
+        return {
+            "id": "like", "type": "TreeLikelihoodModel", "precision": "float64",
+            "tree_model": {"id": "tree", "type": "TimeTreeModel", "newick": "(((A,B),C),D);",
+                           "internal_heights": {"id": "heights", "type": "Parameter", "tensor": heights,
+                                                "dtype": "torch.float64"},
+                           "taxa": {"id": "taxa", "type": "Taxa", "taxa": [
+                               {"id": n, "type": "Taxon", "attributes": {"date": d}}
+                               for n, d in (("A", 0.0), ("B", 1.0), ("C", 4.0), ("D", 5.0))]}},
+            "site_model": {"id": "sm", "type": "WeibullSiteModel", "categories": 4,
+                           "shape": {"id": "s", "type": "Parameter", "tensor": shape, "dtype": "torch.float64"}},
+            "substitution_model": {"id": "jc", "type": "JC69"},
+            "site_pattern": {"id": "sites", "type": "SitePattern", "alignment": {
+                "id": "alignment", "type": "Alignment", "datatype": "nucleotide", "taxa": "taxa",
+                "sequences": [{"taxon": "A", "sequence": "AAG"}, {"taxon": "B", "sequence": "AAC"},
+                              {"taxon": "C", "sequence": "AAC"}, {"taxon": "D", "sequence": "AAT"}]}},
+            "branch_model": {"id": "clock", "type": "StrictClockModel", "tree_model": "tree",
+                             "rate": {"id": "r", "type": "Parameter", "tensor": rate, "dtype": "torch.float64"}},
+        }
+
+    batch = TreeLikelihoodModel.from_json(graph([[10.0, 20.0, 30.0], [100.0, 200.0, 300.0]],
+                                                [[1.0], [1.0]], [[0.01], [0.001]]), {})
+    single = TreeLikelihoodModel.from_json(graph([100.0, 200.0, 300.0], [1.0], [0.001]), {})
+    assert single() == batch()[1]
+
+
+def test_json_gradients_flow(gpu, tiny_newick, tiny_fasta):
+    """Gradients reach JSON parameters (branch lengths, GTR rates, frequencies)."""
+    print("case: JSON gradients", flush=True)
+    from torchtree_amd.treelikelihood_model import TreeLikelihoodModel
+
+    dic = {}
+    data = {
+        "id": "like", "type": "TreeLikelihoodModel", "precision": "float64",
+        "tree_model": {"id": "tree", "type": "UnRootedTreeModel", "newick": tiny_newick,
+                       "branch_lengths": {"id": "branches", "type": "Parameter", "tensor": [0.0],
+                                          "dtype": "torch.float64"},
+                       "keep_branch_lengths": True, "taxa": _taxa(tiny_fasta)},
+        "site_model": {"id": "sm", "type": "ConstantSiteModel"},
+        "substitution_model": {"id": "gtr", "type": "GTR",
+                               "rates": {"id": "rates", "type": "Parameter", "tensor": [1.0, 2.0, 1.0, 1.0, 2.0, 1.0],
+                                         "dtype": "torch.float64"},
+                               "frequencies": {"id": "freqs", "type": "Parameter", "tensor": [0.3, 0.2, 0.2, 0.3],
+                                               "dtype": "torch.float64"}},
+        "site_pattern": {"id": "sp", "type": "SitePattern",
+                         "alignment": {"id": "alignment", "type": "Alignment",
+                                       "datatype": "nucleotide", "file": tiny_fasta, "taxa": "taxa"}},
+    }
+    like = TreeLikelihoodModel.from_json(data, dic)
+    for k in ("branches", "rates", "freqs"):
+        dic[k].tensor.requires_grad_(True)
+    out = like()
+    out.sum().backward()
+    for k in ("branches", "rates", "freqs"):
+        g = dic[k].tensor.grad
+        assert g is not None and torch.isfinite(g).all(), k
+    # central differences on one branch length
+    eps = 1e-6
+    bl = dic["branches"].tensor.detach().clone()
+    def f(x):
+        dic["branches"].tensor = x
+        return float(like())
+    e = torch.zeros_like(bl)
+    e[2] = eps
+    fd = (f(bl + e) - f(bl - e)) / (2 * eps)
+    assert abs(fd - float(dic_grad := like_grad(dic, bl, like))) < 1e-5 * max(1.0, abs(fd))
+
+
+def like_grad(dic, bl, like):
+    dic["branches"].tensor = bl.clone().requires_grad_(True)
+    like().sum().backward()
+    return dic["branches"].tensor.grad[2]

@@ -1,0 +1,150 @@
+"""Host-side logic of the engine (no GPU): topology indexing, compression,
+tip encoding, ratio transform, JSON parsing, and the C ABI library itself."""
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import data as odata
+from torchtree_amd import alignment as al
+from torchtree_amd import tree as ht
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def test_topology_matches_oracle(tiny_newick, flu_newick, flu_fasta):
+    for nwk, taxa in ((tiny_newick, None), (flu_newick, [t for t, _ in odata.read_fasta(flu_fasta)])):
+        a = ht.Topology(nwk, taxa)
+        b = odata.Tree(nwk, taxa)
+        assert a.taxa == b.taxa
+        assert [tuple(map(int, r)) for r in a.postorder] == b.postorder
+        assert np.allclose(a.edge_lengths(), b.edge_lengths())
+        assert np.allclose(a.unrooted_lengths(), b.unrooted_branch_lengths())
+
+
+def test_heights_and_ratio_roundtrip(flu_newick, flu_fasta):
+    taxa = [t for t, _ in odata.read_fasta(flu_fasta)]
+    dates = [float(t.split("_")[-1]) for t in taxa]
+    topo = ht.Topology(flu_newick, taxa)
+    lh = ht.leaf_heights_from_dates(dates)
+    h = topo.heights_from_lengths(lh)
+    ot = odata.Tree(flu_newick, taxa)
+    ot.set_dates(dates)
+    assert np.allclose(h, ot.heights_from_branch_lengths().numpy())
+    bounds = ht.ratio_bounds(topo, lh)
+    ht_ = torch.tensor(h, dtype=torch.float64)
+    r = ht.heights_to_ratios(topo, bounds, ht_)
+    assert torch.all((r[:-1] > 0) & (r[:-1] < 1))
+    back = ht.ratios_to_heights(topo, bounds, r)
+    assert torch.allclose(back, ht_, rtol=1e-12)
+    bl = ht.time_branch_lengths(topo, torch.tensor(lh), ht_)
+    assert torch.allclose(bl, ot.time_branch_lengths(ht_))
+
+
+def test_compress_matches_oracle_order_and_weights(tiny_fasta):
+    seqs = [s for _, s in odata.read_fasta(tiny_fasta)]
+    a, wa = al.compress(seqs)
+    b, wb = odata.compress(seqs)
+    assert [list(x) for x in a] == [list(x) for x in b]
+    assert torch.equal(wa, wb)
+
+
+def test_reference_site_pattern_fixture():
+    """test/test_site_pattern.py:9-20: weights [2, 1], four taxa, two patterns."""
+    cols, w = al.compress_bytes(["AAG", "AAC", "AAC", "AAT"])
+    assert w.tolist() == [2, 1]
+    codes, tbl = al.encode_tips(cols, al.NucleotideType())
+    assert codes.shape == (4, 2) and tbl.shape[1] == 4
+
+
+@pytest.mark.parametrize("dt,chars", [("aa", "ACDEFGHIKLMNPQRSTVWYBZX-?J"), ("codon", None)])
+def test_encodings_match_oracle(dt, chars):
+    rng = np.random.default_rng(0)
+    if dt == "aa":
+        seqs = ["".join(rng.choice(list(chars), 60)) for _ in range(5)]
+        cols, w = al.compress_bytes(seqs)
+        codes, tbl = al.encode_tips(cols, al.AminoAcidType(), use_ambiguities=True)
+        ref, wref = odata.compress_alignment(seqs, odata.amino_acid_partial)
+    else:
+        trip = ["".join(t) for t in __import__("itertools").product("ACGT", repeat=3)] + ["---", "NNN"]
+        seqs = ["".join(rng.choice(trip, 40)) for _ in range(5)]
+        cols, w = al.compress_bytes(seqs, 3)
+        ct = al.CodonType()
+        codes, tbl = al.encode_tips(cols, ct)
+        ref, wref = odata.compress_alignment(
+            seqs, lambda c: odata.codon_partial(c, odata_table(), 61), size=3)
+    assert torch.equal(w, wref)
+    for t in range(len(seqs)):
+        assert torch.equal(tbl[codes[t].long()].t(), ref[t])
+
+
+def odata_table():
+    from oracle.models import UNIVERSAL
+    return UNIVERSAL
+
+
+def test_reference_datatype_fixtures():
+    """test/test_datatype.py:58-89."""
+    nt = al.NucleotideType()
+    assert nt.state("A") == 0 and nt.state("T") == 3 and nt.state("U") == 3
+    assert nt.partial("-") == (1.0,) * 4
+    aa = al.AminoAcidType()
+    assert aa.state("A") == 0 and aa.state("Y") == 19
+    assert aa.partial("/") == (1.0,) * 20
+    ct = al.CodonType()
+    assert ct.state_count == 61
+    assert ct.state("AAA") == 0 and ct.state("TTT") == 60
+    assert ct.state("TAC") == 48 and ct.state("TAT") == 49
+    assert ct.state("TGC") == 54 and ct.state("TGG") == 55
+    assert ct.partial("---") == (1.0,) * 61
+
+
+def test_library_exports_every_header_symbol():
+    from torchtree_amd import _lib
+
+    header = (ROOT / "include" / "hiptree.h").read_text()
+    declared = set(re.findall(r"\b(hx_[a-z0-9_]+)\s*\(", header))
+    lib = _lib.load()
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert declared == set(_lib.EXPORTED)
+
+
+def test_plan_rejects_bad_topology():
+    from torchtree_amd import _lib
+
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    bad = np.array([[3, 0, 1], [4, 3, 1]], dtype=np.int32)  # node 1 has two parents
+    assert lib.hx_plan_create(3, bad.ctypes.data, 0, ctypes.byref(h)) == 5
+    assert b"repeated" in lib.hx_last_error()
+    wrong_root = np.array([[3, 0, 1], [3, 2, 0]], dtype=np.int32)
+    assert lib.hx_plan_create(3, wrong_root.ctypes.data, 0, ctypes.byref(h)) == 5
+    assert lib.hx_error_string(5) == b"bad topology"
+
+
+def test_json_objects_parse(tiny_newick, tiny_fasta):
+    from torchtree_amd import treelikelihood_model as tm
+
+    dic = {}
+    taxa = tm.process_object({"id": "taxa", "type": "Taxa", "taxa": [
+        {"id": t, "type": "Taxon", "attributes": {"date": 0.0}} for t, _ in odata.read_fasta(tiny_fasta)]}, dic)
+    tree = tm.process_object({"id": "tree", "type": "UnRootedTreeModel", "newick": tiny_newick,
+                              "keep_branch_lengths": True, "taxa": "taxa",
+                              "branch_lengths": {"id": "bl", "type": "torchtree.Parameter", "tensor": [0.0]}},
+                             dic)
+    assert tree.branch_lengths().shape == (2 * 10 - 3,)
+    gtr = tm.process_object({"id": "gtr", "type": "torchtree.evolution.substitution_model.GTR",
+                             "rates": {"id": "r", "type": "Parameter", "tensor": [1.0, 2, 1, 1, 2, 1], "dtype": "torch.float64"},
+                             "frequencies": {"id": "f", "type": "Parameter", "full": [4], "value": 0.25, "dtype": "torch.float64"}}, dic)
+    q = gtr.q_norm()
+    assert torch.allclose(-(torch.diagonal(q) * gtr.frequencies()).sum(), torch.tensor(1.0, dtype=torch.float64))
+    w = tm.process_object({"id": "sm", "type": "WeibullSiteModel", "categories": 4,
+                           "shape": {"id": "shape", "type": "Parameter", "tensor": [1.0], "dtype": "torch.float64"}}, dic)
+    r, p = w.rates_probs()
+    assert torch.allclose(r, torch.tensor([0.1457844, 0.5131316, 1.0708310, 2.2702530], dtype=torch.float64), rtol=1e-6)
+    with pytest.raises(tm.JSONParseError):
+        tm.process_object("missing", dic)

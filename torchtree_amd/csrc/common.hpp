@@ -36,7 +36,6 @@ const std::string& get_error();
 constexpr int kBlock = 256;      // threads per block for the pruning kernels
 constexpr int kWave = 64;        // CDNA wavefront
 constexpr int kMaxCodes = 256;   // tip table rows held in LDS
-constexpr int kTargetBlocks = 2048;  // ~8 resident blocks per CU on 256 CUs
 
 inline int next_pow2(int x) {
   int p = 1;
@@ -56,15 +55,30 @@ struct Tiling {
   int tiles_per_chunk;
 };
 
-inline Tiling make_tiling(int N, int C, int K) {
+int wide_nt(int C, int S, int rb);
+
+// S == 4 uses lane-per-(pattern, category) tiles of 256/cp patterns; other S
+// use tiles of wide_nt() patterns handled by (child, category, state) lanes.
+// The chunking depends on N (and the tile shape) only, never on K: every
+// sample's sums run in the same order whatever batch it is evaluated in, so a
+// sample's result is bitwise identical alone and in a batch.
+constexpr int kTargetChunks = 128;
+
+inline Tiling make_tiling(int N, int C, int K, int S, int rb) {
+  (void)K;
   Tiling t;
   t.cp = next_pow2(C < 1 ? 1 : C);
-  t.ppb = kBlock / t.cp;
+  int min_tiles = 1;
+  if (S == 4) {
+    t.ppb = kBlock / t.cp;
+  } else {
+    t.ppb = wide_nt(C, S, rb);
+    min_tiles = 4;
+  }
   t.ntiles = (N + t.ppb - 1) / t.ppb;
-  int want = (kTargetBlocks + K - 1) / K;
-  if (want < 8) want = 8;
-  int nch = want < t.ntiles ? want : t.ntiles;
-  if (nch < 1) nch = 1;
+  int cap = t.ntiles / min_tiles;
+  if (cap < 1) cap = 1;
+  const int nch = cap < kTargetChunks ? cap : kTargetChunks;
   t.tiles_per_chunk = (t.ntiles + nch - 1) / nch;
   t.nchunks = (t.ntiles + t.tiles_per_chunk - 1) / t.tiles_per_chunk;
   return t;

@@ -22,34 +22,11 @@
 // is the per-node max rescaling of calculate_treelikelihood_discrete_rescaled
 // (tree_likelihood.py:186-221) with the divisor rounded to a power of two, so
 // the scaling itself introduces no rounding.
-#include "common.hpp"
+#include "pruning.hpp"
 
 #include <math.h>
 
 namespace hx {
-
-template <typename R>
-struct PassArgs {
-  const int32_t* kids;  // [2T-1][2]
-  int T, N, C, K, B;
-  int ntiles, tiles_per_chunk, nchunks;
-  const R* mats;          // [K][B][C][S][S]
-  const uint8_t* codes;   // [T][N]
-  const R* table;         // [ncodes][S]
-  int ncodes;
-  const R* dense;         // [T][N][S]
-  R* partials;            // [I][K][N][C][S]
-  int32_t* scales;        // [I][K][N]
-  const R* freqs;
-  int freqs_ks;           // stride between samples (0 or S)
-  const R* props;
-  int props_ks;           // stride between samples (0 or C)
-  const double* weights;  // [N]
-  double* rootsums;       // [K][nchunks][1+S+C]
-  R* gpart;               // [B][K][nchunks][C][S][S]
-};
-
-constexpr double kLn2 = 0.69314718055994530942;
 
 template <typename R, int S>
 __device__ __forceinline__ void vload(const R* __restrict__ p, R (&x)[S]) {
@@ -428,7 +405,7 @@ struct WsLayout {
 
 static WsLayout ws_layout(const hx_plan_s* plan, int S, int N, int C, int K, int dtype) {
   const size_t rb = dtype == HX_F64 ? 8 : 4;
-  const Tiling t = make_tiling(N, C, K);
+  const Tiling t = make_tiling(N, C, K, S, (int)rb);
   const size_t I = plan->I, B = 2 * (size_t)plan->T - 2;
   WsLayout w;
   size_t off = 0;
@@ -511,9 +488,14 @@ struct CallArgs {
 
 static int validate(const CallArgs& c) {
   HX_REQUIRE(c.plan != nullptr, HX_ERR_INVALID_ARGUMENT, "null plan");
-  HX_REQUIRE(c.S == 4, HX_ERR_UNSUPPORTED, "this build supports S=4 in the VALU pruning kernels");
+  HX_REQUIRE(c.S >= 2 && c.S <= 64, HX_ERR_UNSUPPORTED, "state count must be 2..64");
   HX_REQUIRE(c.N >= 1 && c.C >= 1 && c.K >= 1, HX_ERR_INVALID_ARGUMENT, "empty dimension");
-  HX_REQUIRE(c.C <= 16, HX_ERR_UNSUPPORTED, "more than 16 rate categories");
+  if (c.S == 4) {
+    HX_REQUIRE(c.C <= 16, HX_ERR_UNSUPPORTED, "more than 16 rate categories");
+  } else {
+    HX_REQUIRE(2 * c.C * (c.S <= 32 ? 32 : 64) <= 512, HX_ERR_UNSUPPORTED,
+               "categories x padded states exceed the dense-state kernels (C*S_pad <= 256)");
+  }
   HX_REQUIRE(c.dtype == HX_F32 || c.dtype == HX_F64, HX_ERR_INVALID_ARGUMENT, "bad dtype");
   HX_REQUIRE(c.tip_kind == HX_TIPS_CODES || c.tip_kind == HX_TIPS_DENSE, HX_ERR_INVALID_ARGUMENT,
              "bad tip kind");
@@ -530,7 +512,7 @@ static int validate(const CallArgs& c) {
 template <typename R>
 static PassArgs<R> make_args(const CallArgs& c) {
   const WsLayout w = ws_layout(c.plan, c.S, c.N, c.C, c.K, c.dtype);
-  const Tiling t = make_tiling(c.N, c.C, c.K);
+  const Tiling t = make_tiling(c.N, c.C, c.K, c.S, c.dtype == HX_F64 ? 8 : 4);
   char* base = static_cast<char*>(c.ws);
   PassArgs<R> a;
   a.kids = c.plan->d_kids;
@@ -566,7 +548,9 @@ static int run_post(const CallArgs& c, double* out_l, double* out_f, double* out
   const int H = (int)plan->post_off.size() - 1;
   for (int l = 0; l < H; ++l) {
     const int off = plan->post_off[l], width = plan->post_off[l + 1] - off;
-    int rc = dispatch_cp<R, 4>(true, a, plan->d_post + off, width, c.tip_kind, l == H - 1, st);
+    int rc = c.S == 4
+                 ? dispatch_cp<R, 4>(true, a, plan->d_post + off, width, c.tip_kind, l == H - 1, st)
+                 : launch_post_wide<R>(a, plan->d_post + off, width, c.tip_kind, l == H - 1, c.S, st);
     if (rc != HX_OK) return rc;
   }
   const int Q = 1 + c.S + c.C;
@@ -584,7 +568,9 @@ static int run_pre(const CallArgs& c, double* out_dmats, hipStream_t st) {
   const int D = (int)plan->pre_off.size() - 1;
   for (int d = 0; d < D; ++d) {
     const int off = plan->pre_off[d], width = plan->pre_off[d + 1] - off;
-    int rc = dispatch_cp<R, 4>(false, a, plan->d_pre + off, width, c.tip_kind, d == 0, st);
+    int rc = c.S == 4
+                 ? dispatch_cp<R, 4>(false, a, plan->d_pre + off, width, c.tip_kind, d == 0, st)
+                 : launch_pre_wide<R>(a, plan->d_pre + off, width, c.tip_kind, d == 0, c.S, st);
     if (rc != HX_OK) return rc;
   }
   const int CSS = c.C * c.S * c.S;

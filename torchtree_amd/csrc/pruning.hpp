@@ -1,0 +1,40 @@
+// Declarations shared by the pruning translation units.
+#pragma once
+
+#include "common.hpp"
+
+namespace hx {
+
+template <typename R>
+struct PassArgs {
+  const int32_t* kids;  // [2T-1][2]
+  int T, N, C, K, B;
+  int ntiles, tiles_per_chunk, nchunks;
+  const R* mats;          // [K][B][C][S][S]
+  const uint8_t* codes;   // [T][N]
+  const R* table;         // [ncodes][S]
+  int ncodes;
+  const R* dense;         // [T][N][S]
+  R* partials;            // [I][K][N][C][S]
+  int32_t* scales;        // [I][K][N]
+  const R* freqs;
+  int freqs_ks;           // stride between samples (0 or S)
+  const R* props;
+  int props_ks;           // stride between samples (0 or C)
+  const double* weights;  // [N]
+  double* rootsums;       // [K][nchunks][1+S+C]
+  R* gpart;               // [B][K][nchunks][C][S][S]
+};
+
+constexpr double kLn2 = 0.69314718055994530942;
+
+// Dense-state (S != 4) launchers, defined in pruning_wide.hip.
+int wide_nt(int C, int S, int rb);
+template <typename R>
+int launch_post_wide(const PassArgs<R>& a, const int32_t* nodes, int width, int tips, bool root,
+                     int S, hipStream_t st);
+template <typename R>
+int launch_pre_wide(const PassArgs<R>& a, const int32_t* nodes, int width, int tips, bool root,
+                    int S, hipStream_t st);
+
+}  // namespace hx

@@ -1,0 +1,565 @@
+"""JSON ``TreeLikelihoodModel`` backed by the HIP engine.
+
+Accepts the object graph of torchtree 1.0.2 for this path and returns the
+same ``[*sample_shape, 1]`` log-likelihood, differentiable with respect to
+every parameter tensor (``requires_grad``):
+
+* ``TreeLikelihoodModel`` (``evolution/tree_likelihood.py:281-450``): keys ``id``,
+  ``tree_model``, ``site_model``, ``substitution_model``, ``site_pattern``, optional
+  ``branch_model``, ``use_ambiguities``, ``use_tip_states``; an extra optional
+  ``precision`` key ("float32" | "float64") picks the partials precision
+  (default: torch's default dtype, as the reference follows it).
+* ``Parameter`` (``core/parameter.py:189``): ``tensor``, ``full``, ``zeros``, ``ones``
+  (+ ``value``, ``dtype``, ``requires_grad``).
+* ``Taxa``/``Taxon``, ``Alignment`` (nucleotide / amino acid / codon / general data
+  types; ``sequences`` or ``file``), ``SitePattern``.
+* ``UnRootedTreeModel`` (``tree_model.py:225-344``), ``TimeTreeModel`` (349-463),
+  ``ReparameterizedTimeTreeModel`` with ``ratios``/``root_height`` (527-707).
+* ``JC69``, ``HKY``, ``GTR`` (``nucleotide.py``), ``GeneralJC69``,
+  ``GeneralSymmetricSubstitutionModel``, ``EmpiricalSubstitutionModel``
+  (``general.py``), ``MG94`` (``codon.py``).
+* ``ConstantSiteModel``, ``InvariantSiteModel``, ``WeibullSiteModel``
+  (``site_model.py``) and ``GammaSiteModel`` (new; torchtree has none).
+* ``StrictClockModel``, ``SimpleClockModel`` (``branch_model.py:38-77``).
+
+Types are matched on the last component of the (possibly dotted) type name,
+as ``core/utils.py:116-125`` resolves registered classes. Objects given as a
+string are looked up by id in ``dic``.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from . import alignment as al
+from . import models as hm
+from .engine import TreeJob
+from .likelihood import TipData
+from .tree import (Topology, heights_to_ratios, leaf_heights_from_dates, ratio_bounds,
+                   ratios_to_heights, time_branch_lengths)
+
+_REGISTRY = {}
+
+
+def register(name):
+    def deco(fn):
+        _REGISTRY[name] = fn
+        return fn
+
+    return deco
+
+
+class JSONParseError(ValueError):
+    pass
+
+
+def process_object(data, dic):
+    if isinstance(data, str):
+        if data not in dic:
+            raise JSONParseError(f"Object with ID `{data}' not found")
+        return dic[data]
+    if not isinstance(data, dict) or "type" not in data:
+        raise JSONParseError(f"cannot build an object from {data!r}")
+    kind = data["type"].split(".")[-1]
+    fn = _REGISTRY.get(kind)
+    if fn is None:
+        raise JSONParseError(f"unknown type `{data['type']}'")
+    obj = fn(data, dic)
+    if data.get("id") is not None:
+        if data["id"] in dic and dic[data["id"]] is not obj:
+            raise JSONParseError(f"Object with ID `{data['id']}' already exists")
+        dic[data["id"]] = obj
+    return obj
+
+
+def _opt(data, key, dic):
+    return process_object(data[key], dic) if key in data else None
+
+
+# ------------------------------------------------------------- parameters
+
+
+class Parameter:
+    """A named tensor whose value models read at call time (like torchtree.Parameter)."""
+
+    def __init__(self, id_, tensor):
+        self.id = id_
+        self.tensor = tensor
+
+    @property
+    def shape(self):
+        return self.tensor.shape
+
+
+_DTYPES = {"torch.float64": torch.float64, "float64": torch.float64, "torch.float32": torch.float32,
+           "float32": torch.float32, "torch.int64": torch.int64, "torch.long": torch.int64}
+
+
+@register("Parameter")
+def _parameter(data, dic):
+    dtype = _DTYPES.get(data.get("dtype")) if "dtype" in data else None
+    if "tensor" in data:
+        t = torch.tensor(data["tensor"], dtype=dtype)
+        if dtype is None and t.is_floating_point():
+            t = t.to(torch.get_default_dtype())
+    elif "full" in data:
+        size = data["full"] if isinstance(data["full"], list) else [data["full"]]
+        t = torch.full(size, data["value"], dtype=dtype or torch.get_default_dtype())
+    elif "zeros" in data:
+        size = data["zeros"] if isinstance(data["zeros"], list) else [data["zeros"]]
+        t = torch.zeros(size, dtype=dtype or torch.get_default_dtype())
+    elif "ones" in data:
+        size = data["ones"] if isinstance(data["ones"], list) else [data["ones"]]
+        t = torch.ones(size, dtype=dtype or torch.get_default_dtype())
+    else:
+        raise JSONParseError(f"Parameter `{data.get('id')}' has no value")
+    if data.get("requires_grad", False):
+        t.requires_grad_(True)
+    return Parameter(data.get("id"), t)
+
+
+# ------------------------------------------------------------ taxa / data
+
+
+class Taxon:
+    def __init__(self, id_, attributes):
+        self.id = id_
+        self.attributes = dict(attributes or {})
+
+    def __getitem__(self, k):
+        return self.attributes[k]
+
+
+@register("Taxon")
+def _taxon(data, dic):
+    return Taxon(data["id"], data.get("attributes", {}))
+
+
+@register("Taxa")
+def _taxa(data, dic):
+    return [process_object(t, dic) for t in data["taxa"]]
+
+
+def _data_type(spec, dic):
+    if isinstance(spec, str):
+        if spec == "nucleotide":
+            return al.NucleotideType()
+        if spec in dic:
+            return dic[spec]
+        raise JSONParseError(f"unknown data type `{spec}'")
+    kind = spec["type"].split(".")[-1]
+    if kind == "NucleotideDataType":
+        dt = al.NucleotideType()
+    elif kind == "AminoAcidDataType":
+        dt = al.AminoAcidType()
+    elif kind == "CodonDataType":
+        dt = al.CodonType(spec.get("genetic_code", "universal"))
+    elif kind == "GeneralDataType":
+        dt = al.GeneralType(spec["codes"], spec.get("ambiguities", {}))
+    else:
+        raise JSONParseError(f"unknown data type `{spec['type']}'")
+    if spec.get("id") is not None:
+        dic[spec["id"]] = dt
+    return dt
+
+
+for _k in ("NucleotideDataType", "AminoAcidDataType", "CodonDataType", "GeneralDataType"):
+    register(_k)(lambda data, dic: _data_type(data, dic))
+
+
+class Alignment:
+    def __init__(self, sequences, taxa, data_type):
+        order = {t.id: i for i, t in enumerate(taxa)}
+        self.sequences = sorted(sequences, key=lambda s: order[s[0]])
+        self.taxa = taxa
+        self.data_type = data_type
+
+
+@register("Alignment")
+def _alignment(data, dic):
+    taxa = process_object(data["taxa"], dic)
+    dt = _data_type(data["datatype"], dic)
+    if "sequences" in data:
+        seqs = [(s["taxon"], s["sequence"]) for s in data["sequences"]]
+    elif "file" in data:
+        seqs = al.read_fasta(data["file"])
+    else:
+        raise JSONParseError("sequences or file should be specified in Alignment")
+    return Alignment(seqs, taxa, dt)
+
+
+class SitePattern:
+    def __init__(self, alignment):
+        self.alignment = alignment
+
+    def compressed(self):
+        seqs = [s for _, s in self.alignment.sequences]
+        return al.compress_bytes(seqs, self.alignment.data_type.size)
+
+
+@register("SitePattern")
+def _site_pattern(data, dic):
+    if data.get("indices"):
+        raise JSONParseError("SitePattern `indices' is not supported by this engine")
+    return SitePattern(process_object(data["alignment"], dic))
+
+
+# ------------------------------------------------------------ tree models
+
+
+class UnRootedTreeModel:
+    def __init__(self, topo, taxa, branch_lengths: Parameter):
+        self.topo, self.taxa, self._bl = topo, taxa, branch_lengths
+
+    def branch_lengths(self):
+        return self._bl.tensor
+
+    @property
+    def sample_shape(self):
+        return tuple(self._bl.tensor.shape[:-1])
+
+
+def _newick(data):
+    if "newick" in data:
+        return data["newick"]
+    if "file" in data:
+        with open(data["file"]) as fp:
+            return fp.read().strip()
+    raise JSONParseError("Tree model requires a file or newick element to be specified")
+
+
+@register("UnRootedTreeModel")
+def _unrooted(data, dic):
+    taxa = process_object(data["taxa"], dic)
+    topo = Topology(_newick(data), [t.id for t in taxa])
+    bl = process_object(data["branch_lengths"], dic)
+    if data.get("keep_branch_lengths"):
+        bl.tensor = torch.tensor(topo.unrooted_lengths(), dtype=bl.tensor.dtype)
+    return UnRootedTreeModel(topo, taxa, bl)
+
+
+class TimeTreeModel:
+    """Branch lengths from node heights (tree_model.py:407-424)."""
+
+    def __init__(self, topo, taxa, internal: Parameter, ratios=False):
+        self.topo, self.taxa, self._p = topo, taxa, internal
+        self.leaf_heights = torch.tensor(leaf_heights_from_dates([t["date"] for t in taxa]))
+        self.ratios = ratios
+        self.bounds = ratio_bounds(topo, self.leaf_heights.numpy())
+
+    def internal_heights(self):
+        x = self._p.tensor
+        return ratios_to_heights(self.topo, self.bounds, x) if self.ratios else x
+
+    def branch_lengths(self):
+        return time_branch_lengths(self.topo, self.leaf_heights, self.internal_heights())
+
+    @property
+    def sample_shape(self):
+        return tuple(self._p.tensor.shape[:-1])
+
+
+@register("TimeTreeModel")
+def _time_tree(data, dic):
+    taxa = process_object(data["taxa"], dic)
+    topo = Topology(_newick(data), [t.id for t in taxa])
+    p = process_object(data["internal_heights"], dic)
+    model = TimeTreeModel(topo, taxa, p)
+    if data.get("keep_branch_lengths"):
+        h = topo.heights_from_lengths(model.leaf_heights.numpy())
+        p.tensor = torch.tensor(h, dtype=p.tensor.dtype)
+    return model
+
+
+@register("ReparameterizedTimeTreeModel")
+def _reparam_tree(data, dic):
+    taxa = process_object(data["taxa"], dic)
+    topo = Topology(_newick(data), [t.id for t in taxa])
+    if "shifts" in data:
+        raise JSONParseError("ReparameterizedTimeTreeModel with shifts is not supported")
+    ratios = process_object(data["ratios"], dic)
+    root = process_object(data["root_height"], dic)
+
+    class _Cat(Parameter):
+        @property
+        def tensor(self):
+            return torch.cat((ratios.tensor, root.tensor), -1)
+
+        @tensor.setter
+        def tensor(self, value):
+            ratios.tensor = value[..., :-1]
+            root.tensor = value[..., -1:]
+
+    cat = _Cat.__new__(_Cat)
+    cat.id = None
+    model = TimeTreeModel(topo, taxa, cat, ratios=True)
+    if data.get("keep_branch_lengths"):
+        h = torch.tensor(topo.heights_from_lengths(model.leaf_heights.numpy()),
+                         dtype=ratios.tensor.dtype)
+        cat.tensor = heights_to_ratios(topo, model.bounds, h)
+    return model
+
+
+# ---------------------------------------------------- substitution models
+
+
+class SubstModel:
+    """Normalised Q [..., S, S] and frequencies [..., S] read at call time."""
+
+    def __init__(self, S, qfn, ffn, shape_fn):
+        self.state_count = S
+        self._q = qfn
+        self._f = ffn
+        self._shape = shape_fn
+
+    def frequencies(self):
+        return self._f()
+
+    def q_norm(self):
+        f = self.frequencies()
+        return hm.normalize(self._q(), f)
+
+    @property
+    def sample_shape(self):
+        return self._shape()
+
+
+def _shape_of(*params):
+    shapes = [tuple(p.tensor.shape[:-1]) for p in params if p is not None]
+    return max(shapes, key=len) if shapes else ()
+
+
+@register("JC69")
+def _jc69(data, dic):
+    pi = torch.full((4,), 0.25, dtype=torch.float64)
+    return SubstModel(4, lambda: hm.gtr_q(torch.ones(6, dtype=torch.float64), pi), lambda: pi,
+                      lambda: ())
+
+
+@register("GeneralJC69")
+def _gjc69(data, dic):
+    S = int(data["state_count"])
+    pi = torch.full((S,), 1.0 / S, dtype=torch.float64)
+    ones = torch.ones(S * (S - 1) // 2, dtype=torch.float64)
+    return SubstModel(S, lambda: hm.q_from_exchangeabilities(ones, pi), lambda: pi, lambda: ())
+
+
+@register("HKY")
+def _hky(data, dic):
+    kappa = process_object(data["kappa"], dic)
+    freqs = process_object(data["frequencies"], dic)
+    return SubstModel(4, lambda: hm.hky_q(kappa.tensor, freqs.tensor), lambda: freqs.tensor,
+                      lambda: _shape_of(kappa, freqs))
+
+
+@register("GTR")
+def _gtr(data, dic):
+    rates = process_object(data["rates"], dic)
+    freqs = process_object(data["frequencies"], dic)
+    return SubstModel(4, lambda: hm.gtr_q(rates.tensor, freqs.tensor), lambda: freqs.tensor,
+                      lambda: _shape_of(rates, freqs))
+
+
+@register("GeneralSymmetricSubstitutionModel")
+def _general(data, dic):
+    dt = _data_type(data["data_type"], dic)
+    rates = process_object(data["rates"], dic)
+    freqs = process_object(data["frequencies"], dic)
+    S = dt.state_count
+    if "mapping" not in data:
+        mapping = torch.arange(S * (S - 1) // 2)
+    elif isinstance(data["mapping"], list):
+        mapping = torch.tensor(data["mapping"])
+    else:
+        mapping = process_object(data["mapping"], dic).tensor.long()
+    return SubstModel(S, lambda: hm.general_symmetric_q(rates.tensor, mapping, freqs.tensor),
+                      lambda: freqs.tensor, lambda: _shape_of(rates, freqs))
+
+
+@register("EmpiricalSubstitutionModel")
+def _empirical(data, dic):
+    rates = torch.tensor(data["rates"], dtype=torch.float64)
+    freqs = torch.tensor(data["frequencies"], dtype=torch.float64)
+    S = freqs.shape[-1]
+    return SubstModel(S, lambda: hm.q_from_exchangeabilities(rates, freqs), lambda: freqs,
+                      lambda: ())
+
+
+@register("MG94")
+def _mg94(data, dic):
+    dt = _data_type(data["data_type"], dic)
+    alpha = process_object(data["alpha"], dic)
+    beta = process_object(data["beta"], dic)
+    kappa = process_object(data["kappa"], dic)
+    freqs = process_object(data["frequencies"], dic)
+    flags = hm.mg94_flags(dt.table)
+
+    def q():
+        a, b, k = torch.broadcast_tensors(alpha.tensor, beta.tensor, kappa.tensor)
+        return hm.mg94_q(a, b, k, freqs.tensor, flags)
+
+    return SubstModel(dt.state_count, q, lambda: freqs.tensor,
+                      lambda: _shape_of(alpha, beta, kappa, freqs))
+
+
+# ------------------------------------------------------------ site models
+
+
+class SiteModel:
+    def __init__(self, fn, shape_fn):
+        self._fn = fn
+        self._shape = shape_fn
+
+    def rates_probs(self):
+        return self._fn()
+
+    @property
+    def sample_shape(self):
+        return self._shape()
+
+
+@register("ConstantSiteModel")
+def _constant(data, dic):
+    mu = _opt(data, "mu", dic)
+
+    def fn():
+        r = mu.tensor if mu is not None else torch.ones(1, dtype=torch.float64)
+        return r, torch.ones_like(r)
+
+    return SiteModel(fn, lambda: _shape_of(mu))
+
+
+@register("InvariantSiteModel")
+def _invariant(data, dic):
+    inv = process_object(data["invariant"], dic)
+    mu = _opt(data, "mu", dic)
+    return SiteModel(lambda: hm.invariant_rates(inv.tensor, mu.tensor if mu else None),
+                     lambda: _shape_of(inv, mu))
+
+
+@register("WeibullSiteModel")
+def _weibull(data, dic):
+    shape = process_object(data["shape"], dic)
+    cats = int(data["categories"])
+    inv = _opt(data, "invariant", dic)
+    mu = _opt(data, "mu", dic)
+    return SiteModel(lambda: hm.weibull_rates(shape.tensor, cats, inv.tensor if inv else None,
+                                              mu.tensor if mu else None),
+                     lambda: _shape_of(shape, inv, mu))
+
+
+@register("GammaSiteModel")
+def _gamma(data, dic):
+    shape = process_object(data["shape"], dic)
+    cats = int(data["categories"])
+
+    def fn():
+        if shape.tensor.requires_grad:
+            raise JSONParseError("GammaSiteModel shape gradients are not provided")
+        r, p = hm.gamma_rates(float(shape.tensor.reshape(-1)[0]), cats)
+        return r, p
+
+    return SiteModel(fn, lambda: ())
+
+
+# ---------------------------------------------------------- branch models
+
+
+class ClockModel:
+    def __init__(self, rate: Parameter, B, strict):
+        self._r, self.B, self.strict = rate, B, strict
+
+    def rates(self):
+        r = self._r.tensor
+        return r.expand(r.shape[:-1] + (self.B,)) if self.strict else r
+
+    @property
+    def sample_shape(self):
+        return _shape_of(self._r)
+
+
+@register("StrictClockModel")
+def _strict(data, dic):
+    tree = process_object(data["tree_model"], dic)
+    return ClockModel(process_object(data["rate"], dic), tree.topo.B, True)
+
+
+@register("SimpleClockModel")
+def _simple(data, dic):
+    tree = process_object(data["tree_model"], dic)
+    return ClockModel(process_object(data["rate"], dic), tree.topo.B, False)
+
+
+# ------------------------------------------------------- the likelihood
+
+
+class TreeLikelihoodModel:
+    """HIP-backed ``TreeLikelihoodModel``: ``model()`` -> log-likelihood [*sample_shape, 1]."""
+
+    def __init__(self, id_, site_pattern, tree_model, subst_model, site_model, clock_model=None,
+                 use_ambiguities=False, use_tip_states=False, precision=None, device=None):
+        self.id = id_
+        self.site_pattern, self.tree_model = site_pattern, tree_model
+        self.subst_model, self.site_model, self.clock_model = subst_model, site_model, clock_model
+        if device is None:
+            device = torch.device("cuda", torch.cuda.current_device())
+        self.device = torch.device(device)
+        if precision is None:
+            precision = torch.get_default_dtype()
+        self.dtype = _DTYPES[precision] if isinstance(precision, str) else precision
+        cols, weights = site_pattern.compressed()
+        dt = site_pattern.alignment.data_type
+        codes, table = al.encode_tips(cols, dt, use_ambiguities, use_tip_states)
+        S = dt.state_count
+        topo = tree_model.topo
+        tips = TipData(0, codes.contiguous().to(self.device), table.to(self.device), S,
+                       codes.shape[1], topo.T)
+        self.weights = weights
+        self.job = TreeJob([tuple(int(x) for x in r) for r in topo.postorder], tips, weights, S,
+                           dtype=self.dtype, device=self.device)
+
+    @property
+    def sample_shape(self):
+        shapes = [self.tree_model.sample_shape, self.subst_model.sample_shape,
+                  self.site_model.sample_shape]
+        if self.clock_model is not None:
+            shapes.append(self.clock_model.sample_shape)
+        return max(shapes, key=len)
+
+    def __call__(self):
+        shape = tuple(self.sample_shape)
+        if len(shape) > 1:
+            raise ValueError("sample shapes with more than one dimension are not supported")
+        K = shape[0] if shape else 1
+        bl = self.tree_model.branch_lengths()
+        if self.clock_model is None:
+            bl = bl.expand((K,) + bl.shape[-1:])
+            bls = torch.cat((bl, torch.zeros(K, 1, dtype=bl.dtype)), -1)
+        else:
+            bls = (self.clock_model.rates() * bl).expand((K,) + bl.shape[-1:])
+        rates, probs = self.site_model.rates_probs()
+        rates = rates.reshape(-1, rates.shape[-1]).expand(K, -1) if rates.dim() > 1 else rates.unsqueeze(0)
+        probs = probs.reshape(-1, probs.shape[-1]).expand(K, -1) if probs.dim() > 1 else probs.unsqueeze(0)
+        q = self.subst_model.q_norm()
+        f = self.subst_model.frequencies()
+        if q.dim() == 3 and q.shape[0] == 1:
+            q, f = q[0], f.reshape(-1, f.shape[-1])[0]
+        ll = self.job.loglik(bls.to(torch.float64), rates.to(torch.float64),
+                             probs.to(torch.float64), q.to(torch.float64), f.to(torch.float64))
+        out = ll.to(torch.get_default_dtype()).cpu()
+        return out.reshape(shape + (1,))
+
+    @classmethod
+    def from_json(cls, data, dic):
+        tree = process_object(data["tree_model"], dic)
+        site = process_object(data["site_model"], dic)
+        subst = process_object(data["substitution_model"], dic)
+        pattern = process_object(data["site_pattern"], dic)
+        clock = _opt(data, "branch_model", dic)
+        return cls(data["id"], pattern, tree, subst, site, clock, data.get("use_ambiguities", False),
+                   data.get("use_tip_states", False), data.get("precision"))
+
+
+register("TreeLikelihoodModel")(lambda data, dic: TreeLikelihoodModel.from_json(data, dic))
