@@ -76,8 +76,10 @@ int hx_plan_destroy(hx_plan_t plan);
 /* levels: post-order height levels; depth: pre-order depth levels. */
 int hx_plan_info(hx_plan_t plan, int* post_levels, int* pre_levels, int* max_width);
 
-/* Device workspace needed by hx_loglik / hx_loglik_grad_mats for one call. */
-size_t hx_workspace_bytes(hx_plan_t plan, int S, int N, int C, int K, int dtype);
+/* Device workspace for one call. grad_mode: HX_GRAD_NONE (hx_loglik only),
+ * HX_GRAD_MATS (then hx_loglik_grad_mats), HX_GRAD_TAU (then hx_loglik_grad_tau). */
+enum { HX_GRAD_NONE = 0, HX_GRAD_MATS = 1, HX_GRAD_TAU = 2 };
+size_t hx_workspace_bytes(hx_plan_t plan, int S, int N, int C, int K, int dtype, int grad_mode);
 
 /* --------------------------------------------------------- likelihood */
 /* Post-order pruning with exact power-of-two rescaling of every node,
@@ -107,6 +109,23 @@ int hx_loglik_grad_mats(hx_plan_t plan, int S, int N, int C, int K, int dtype,
                         void* workspace, size_t workspace_bytes,
                         double* out_dmats, hx_stream_t stream);
 
+/* Pre-order adjoint reduced to branch-length scale: out_dtau [K][B][C] double
+ * = d loglik / d tau with tau = bls[k][b] * rates[k][c], i.e. the contraction
+ * of d loglik / d mats with dP/dtau (`mats_dtau`, from hx_pmatrix_dtau, same
+ * layout and dtype as mats). Needs a workspace sized for HX_GRAD_TAU; the
+ * per-chunk state is C doubles per branch instead of C*S*S values, which is
+ * what keeps 61-state requests with many samples inside a bounded workspace.
+ * Replaces the same autograd backward as hx_loglik_grad_mats, projected on
+ * the branch-length and rate-category directions (tree_likelihood.py:346,
+ * mats = p_t(bls * rates)). */
+int hx_loglik_grad_tau(hx_plan_t plan, int S, int N, int C, int K, int dtype,
+                       int tip_kind, const void* tips, const void* tip_table, int ncodes,
+                       const double* weights, const void* mats, const void* mats_dtau,
+                       const void* freqs, int freqs_batched,
+                       const void* props, int props_batched,
+                       void* workspace, size_t workspace_bytes,
+                       double* out_dtau, hx_stream_t stream);
+
 /* ------------------------------------------------------------ P matrices */
 /* mats[k][b][c] = L_e diag(exp(lambda_e * bls[k][b] * rates[k][c])) R_e with
  * e = k if eigen_batched else 0. lambda [E][S], left [E][S][S] (Pi^-1/2 V),
@@ -116,6 +135,14 @@ int hx_pmatrix(int S, int K, int B, int C, int eigen_batched,
                const double* lambda, const double* left, const double* right,
                const double* bls, const double* rates, int rates_batched,
                void* mats, int dtype, hx_stream_t stream);
+
+/* dP/dtau = L_e diag(lambda_e exp(lambda_e tau)) R_e, same arguments and
+ * layout as hx_pmatrix (the derivative of SymmetricSubstitutionModel.p_t,
+ * abstract.py:64-76, with respect to its branch-length argument). */
+int hx_pmatrix_dtau(int S, int K, int B, int C, int eigen_batched,
+                    const double* lambda, const double* left, const double* right,
+                    const double* bls, const double* rates, int rates_batched,
+                    void* mats_dtau, int dtype, hx_stream_t stream);
 
 /* Chain rule from dmats = d loglik / d mats [K][B][C][S][S] (double) to
  *   out_dbls  [K][B]      d loglik / d bls
@@ -131,6 +158,34 @@ int hx_pmatrix_grad(int S, int K, int B, int C, int eigen_batched,
                     const double* bls, const double* rates, int rates_batched,
                     const double* dmats, double* out_dbls, double* out_drates,
                     double* out_dq, void* scratch, hx_stream_t stream);
+
+/* ------------------------------------------------------------ resident jobs */
+/* A job keeps one alignment resident on a device for repeated requests:
+ * uint8 tip codes [T][N] and their table [ncodes][S] (host pointers, copied
+ * and cut into slices of `slice_patterns` patterns), weights [N], the FP64
+ * eigensystem (lambda [S], left [S][S], right [S][S] as in hx_pmatrix),
+ * category rates and proportions [C] and frequencies [S] (host, double).
+ * No reference counterpart as such: TreeLikelihoodModel keeps its tips in
+ * the model object (tree_likelihood.py:293-311) and recomputes every input
+ * per call; a job is the device-resident form of that state for serving. */
+typedef struct hx_job_s* hx_job_t;
+int hx_job_create(hx_plan_t plan, int S, int N, int C, int dtype, int slice_patterns,
+                  const uint8_t* codes, const double* table, int ncodes, const double* weights,
+                  const double* lambda, const double* left, const double* right,
+                  const double* rates, const double* props, const double* freqs, int device,
+                  hx_job_t* job);
+int hx_job_destroy(hx_job_t job);
+size_t hx_job_resident_bytes(hx_job_t job);
+/* Workspace of one hx_job_eval with K samples (with or without the gradient). */
+size_t hx_job_workspace_bytes(hx_job_t job, int K, int want_grad);
+/* One request: bls [K][B] (device, double; unrooted trees carry 0 in slot
+ * 2T-3) -> out_loglik [K] and, if out_dbls is not NULL, out_dbls [K][B]
+ * (device, double). Pattern slices run in order on `stream`; results do not
+ * depend on other requests running concurrently on other streams. This is
+ * TreeLikelihoodModel._call (tree_likelihood.py:313-356) plus the backward
+ * with respect to branch lengths. */
+int hx_job_eval(hx_job_t job, int K, const double* bls, double* out_loglik, double* out_dbls,
+                void* workspace, size_t workspace_bytes, hx_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,89 @@
+"""Served requests on the GPU: results are bitwise the same alone and in a
+concurrent mix, pattern slicing gives the unsliced answer, and an FP64
+job matches the oracle."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import models as omodels
+from oracle import pruning as opruning
+
+pytestmark = pytest.mark.gpu
+F64 = torch.float64
+
+
+def _jobs(gpu, dtype, slice_for=None):
+    from torchtree_amd import serving, synthetic
+
+    rng = np.random.default_rng(3)
+    specs = [synthetic.random_job_spec(k, T, N, rng, name=f"{k}{T}")
+             for k, T, N in (("dna", 40, 700), ("dna", 17, 150), ("aa", 24, 300),
+                             ("codon", 12, 90), ("dna", 90, 2500))]
+    jobs = []
+    for sp in specs:
+        post = [tuple(int(x) for x in r) for r in sp["topo"].postorder]
+        sl = slice_for(sp) if slice_for else 0
+        jobs.append(serving.ResidentJob(sp["name"], post, sp["codes"], sp["table"], sp["weights"],
+                                        sp["q_norm"], sp["freqs"], sp["rates"], sp["props"],
+                                        dtype=dtype, slice_patterns=sl, device=gpu.index or 0,
+                                        kind=sp["kind"]))
+    return specs, jobs
+
+
+def test_alone_equals_mix_and_slicing(gpu):
+    print("case: serving alone == mix", flush=True)
+    from torchtree_amd import serving, synthetic
+
+    specs, jobs = _jobs(gpu, torch.float32)
+    stream = synthetic.request_stream(30, len(jobs), rate=2000.0, seed=9, ks=(1, 3, 8))
+    def requests():
+        return [serving.Request(i, jobs[j], synthetic.draw_bls(specs[j], K, s), arrival=t)
+                for i, (j, K, s, t) in enumerate(stream)]
+    alone = requests()
+    serving.run_sequential(alone, device=gpu.index or 0)
+    srv = serving.Server(32 << 30, n_streams=4, ws_bytes_per_stream=4 << 30, device=gpu.index or 0)
+    for jb in jobs:
+        srv.register(jb.name, jb)
+    mixed = requests()
+    stats = srv.run(mixed)
+    assert stats["requests"] == len(mixed)
+    for a, b in zip(alone, mixed):
+        assert torch.equal(a.loglik, b.loglik), a.rid
+        assert torch.equal(a.dbls, b.dbls), a.rid
+    # pattern slices of 64: per-pattern log-likelihoods are unchanged and summed in
+    # FP64, so logL agrees to reassociation; gradients accumulate FP32 partial sums
+    # per chunk, so they agree to FP32 accuracy (1e-5, the engine's FP32 target)
+    specs2, sliced = _jobs(gpu, torch.float32, slice_for=lambda sp: 64)
+    again = [serving.Request(r.rid, sliced[jobs.index(r.job)], r.bls) for r in alone]
+    serving.run_sequential(again, device=gpu.index or 0)
+    for a, b in zip(alone, again):
+        assert torch.allclose(a.loglik, b.loglik, rtol=1e-12, atol=0)
+        assert float((a.dbls - b.dbls).norm() / a.dbls.norm()) < 1e-5
+
+
+@pytest.mark.parametrize("dtype,rl,rg", [(F64, 1e-12, 1e-9), (torch.float32, 1e-6, 1e-5)])
+def test_served_matches_oracle(gpu, dtype, rl, rg):
+    print(f"case: served vs oracle {dtype}", flush=True)
+    from torchtree_amd import serving, synthetic
+
+    specs, jobs = _jobs(gpu, dtype)
+    for sp, jb in zip(specs, jobs):
+        bls = synthetic.draw_bls(sp, 2, 11)
+        req = serving.Request(0, jb, bls)
+        serving.run_sequential([req], device=gpu.index or 0)
+        b = bls.clone().requires_grad_(True)
+        tau = b.unsqueeze(-1) * torch.as_tensor(sp["rates"]).reshape(1, 1, -1)
+        mats = omodels.p_t_eigh(sp["q_norm"], torch.as_tensor(sp["freqs"]), tau)
+        tbl = torch.as_tensor(sp["table"])
+        tips = [tbl[torch.as_tensor(c).long()].t().contiguous() for c in sp["codes"]]
+        post = [tuple(int(x) for x in r) for r in sp["topo"].postorder]
+        ref = opruning.treelik_discrete_rescaled(tips + [None] * (len(tips) - 1),
+                                                 torch.as_tensor(sp["weights"]), post, mats,
+                                                 torch.as_tensor(sp["freqs"]).reshape(1, -1),
+                                                 torch.as_tensor(sp["props"]).reshape(-1, 1, 1)).reshape(-1)
+        ref.sum().backward()
+        rel = float(((req.loglik.cpu() - ref.detach()).abs() / ref.detach().abs()).max())
+        assert rel < rl, (sp["name"], rel)
+        g = b.grad[:, :-1]  # slot 2T-3 is a padded (fixed) length
+        err = float((req.dbls.cpu()[:, :-1] - g).norm() / g.norm())
+        assert err < rg, (sp["name"], err)

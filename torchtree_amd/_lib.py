@@ -34,7 +34,7 @@ _SIGNATURES = {
     "hx_plan_destroy": (_c_int, [_vp]),
     "hx_plan_info": (_c_int, [_vp, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int),
                               ctypes.POINTER(_c_int)]),
-    "hx_workspace_bytes": (_c_size, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int]),
+    "hx_workspace_bytes": (_c_size, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int]),
     "hx_loglik": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int,
                            _c_int, _vp, _vp, _c_int,
                            _dp, _vp, _vp, _c_int, _vp, _c_int,
@@ -43,13 +43,30 @@ _SIGNATURES = {
                                      _c_int, _vp, _vp, _c_int,
                                      _dp, _vp, _vp, _c_int, _vp, _c_int,
                                      _vp, _c_size, _dp, _vp]),
+    "hx_loglik_grad_tau": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int,
+                                    _c_int, _vp, _vp, _c_int,
+                                    _dp, _vp, _vp, _vp, _c_int, _vp, _c_int,
+                                    _vp, _c_size, _dp, _vp]),
     "hx_pmatrix": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int,
                             _dp, _dp, _dp, _dp, _dp, _c_int, _vp, _c_int, _vp]),
+    "hx_pmatrix_dtau": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int,
+                                 _dp, _dp, _dp, _dp, _dp, _c_int, _vp, _c_int, _vp]),
     "hx_pmatrix_grad_scratch_bytes": (_c_size, [_c_int, _c_int, _c_int, _c_int]),
     "hx_pmatrix_grad": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int,
                                  _dp, _dp, _dp, _dp, _dp, _c_int,
                                  _dp, _dp, _dp, _dp, _vp, _vp]),
+    "hx_job_create": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int,
+                               _vp, _vp, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_int,
+                               ctypes.POINTER(_vp)]),
+    "hx_job_destroy": (_c_int, [_vp]),
+    "hx_job_resident_bytes": (_c_size, [_vp]),
+    "hx_job_workspace_bytes": (_c_size, [_vp, _c_int, _c_int]),
+    "hx_job_eval": (_c_int, [_vp, _c_int, _dp, _dp, _dp, _vp, _c_size, _vp]),
 }
+
+HX_GRAD_NONE = 0
+HX_GRAD_MATS = 1
+HX_GRAD_TAU = 2
 
 EXPORTED = tuple(_SIGNATURES)
 

@@ -1,0 +1,272 @@
+// Resident likelihood jobs for the serving loop.
+//
+// A job keeps one alignment's tip codes (cut into pattern slices), weights,
+// topology plan, FP64 eigensystem, category rates/proportions and
+// frequencies in device memory. hx_job_eval runs one loglik (+ branch-length
+// gradient) request of K samples on a caller-given stream and workspace:
+//   P matrices once, then for each pattern slice post-order (+ pre-order),
+//   accumulating log-likelihood and d loglik / d mats over slices in a fixed
+//   order, then the chain rule to d loglik / d branch lengths.
+// Pattern slices bound the workspace whatever N is; the sums are exact
+// because patterns are independent (TreeLikelihoodModel._call,
+// tree_likelihood.py:313-356, evaluates all N at once).
+#include "common.hpp"
+
+#include <algorithm>
+#include <vector>
+
+struct hx_job_s {
+  hx_plan_t plan = nullptr;
+  int device = 0;
+  int S = 0, N = 0, C = 0, dtype = HX_F32, T = 0, B = 0;
+  int slice = 0, nslices = 0;
+  int ncodes = 0;
+  std::vector<int> slice_n;        // patterns per slice
+  void* d_buf = nullptr;           // one device allocation for everything resident
+  std::vector<uint8_t*> d_codes;   // per slice [T][n]
+  std::vector<double*> d_weights;  // per slice [n]
+  void* d_table = nullptr;         // [ncodes][S] in dtype
+  double* d_lambda = nullptr;      // [S]
+  double* d_left = nullptr;        // [S][S]
+  double* d_right = nullptr;       // [S][S]
+  double* d_rates = nullptr;       // [C]
+  void* d_props = nullptr;         // [C] in dtype
+  void* d_freqs = nullptr;         // [S] in dtype
+  size_t resident = 0;
+};
+
+namespace hx {
+
+__global__ void k_fill0(double* p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+__global__ void k_accum(double* __restrict__ dst, const double* __restrict__ src, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] += src[i];
+}
+
+static void fill0(double* p, size_t n, hipStream_t st) {
+  if (n) k_fill0<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(p, n);
+}
+
+static void accum(double* dst, const double* src, size_t n, hipStream_t st) {
+  if (n) k_accum<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(dst, src, n);
+}
+
+__global__ void k_dbls(int K, int B, int C, const double* __restrict__ rates,
+                       const double* __restrict__ dtau, double* __restrict__ dbls) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)K * B) return;
+  double t = 0.0;
+  for (int c = 0; c < C; ++c) t = fma(rates[c], dtau[idx * C + c], t);
+  dbls[idx] = t;
+}
+
+struct JobWs {
+  size_t mats, mats_dt, prune, prune_bytes, loglik_tmp, dtau_tmp, dtau_acc, total;
+};
+
+static JobWs job_ws(const hx_job_s* j, int K, int grad) {
+  const size_t rb = j->dtype == HX_F64 ? 8 : 4;
+  const size_t nm = (size_t)K * j->B * j->C * j->S * j->S;
+  const size_t nt = (size_t)K * j->B * j->C;
+  JobWs w{};
+  size_t off = 0;
+  w.mats = off;
+  off = align_up(off + nm * rb, 256);
+  if (grad) {
+    w.mats_dt = off;
+    off = align_up(off + nm * rb, 256);
+  }
+  w.prune = off;
+  w.prune_bytes = hx_workspace_bytes(j->plan, j->S, j->slice, j->C, K, j->dtype,
+                                     grad ? HX_GRAD_TAU : HX_GRAD_NONE);
+  off = align_up(off + w.prune_bytes, 256);
+  w.loglik_tmp = off;
+  off = align_up(off + (size_t)K * 8, 256);
+  if (grad) {
+    w.dtau_tmp = off;
+    off = align_up(off + nt * 8, 256);
+    w.dtau_acc = off;
+    off = align_up(off + nt * 8, 256);
+  }
+  w.total = off;
+  return w;
+}
+
+}  // namespace hx
+
+extern "C" {
+
+int hx_job_create(hx_plan_t plan, int S, int N, int C, int dtype, int slice_patterns,
+                  const uint8_t* codes, const double* table, int ncodes, const double* weights,
+                  const double* lambda, const double* left, const double* right,
+                  const double* rates, const double* props, const double* freqs, int device,
+                  hx_job_t* out) {
+  HX_REQUIRE(plan && codes && table && weights && lambda && left && right && rates && props &&
+                 freqs && out,
+             HX_ERR_INVALID_ARGUMENT, "null argument");
+  HX_REQUIRE(S >= 2 && S <= 64 && N >= 1 && C >= 1 && ncodes >= 1 && ncodes <= hx::kMaxCodes,
+             HX_ERR_INVALID_ARGUMENT, "bad dimensions");
+  HX_REQUIRE(dtype == HX_F32 || dtype == HX_F64, HX_ERR_INVALID_ARGUMENT, "bad dtype");
+  auto* j = new hx_job_s();
+  j->plan = plan;
+  j->device = device;
+  j->S = S;
+  j->N = N;
+  j->C = C;
+  j->dtype = dtype;
+  j->T = plan->T;
+  j->B = 2 * plan->T - 2;
+  j->ncodes = ncodes;
+  j->slice = slice_patterns > 0 ? std::min(slice_patterns, N) : N;
+  j->nslices = (N + j->slice - 1) / j->slice;
+  const size_t rb = dtype == HX_F64 ? 8 : 4;
+  // host staging of the resident image
+  std::vector<size_t> off_codes(j->nslices), off_w(j->nslices);
+  size_t off = 0;
+  for (int s = 0; s < j->nslices; ++s) {
+    const int n = std::min(j->slice, N - s * j->slice);
+    j->slice_n.push_back(n);
+    off_codes[s] = off;
+    off = hx::align_up(off + (size_t)j->T * n, 256);
+    off_w[s] = off;
+    off = hx::align_up(off + (size_t)n * 8, 256);
+  }
+  const size_t o_table = off;
+  off = hx::align_up(off + (size_t)ncodes * S * rb, 256);
+  const size_t o_lam = off;
+  off = hx::align_up(off + (size_t)S * 8, 256);
+  const size_t o_left = off;
+  off = hx::align_up(off + (size_t)S * S * 8, 256);
+  const size_t o_right = off;
+  off = hx::align_up(off + (size_t)S * S * 8, 256);
+  const size_t o_rates = off;
+  off = hx::align_up(off + (size_t)C * 8, 256);
+  const size_t o_props = off;
+  off = hx::align_up(off + (size_t)C * rb, 256);
+  const size_t o_freqs = off;
+  off = hx::align_up(off + (size_t)S * rb, 256);
+  std::vector<uint8_t> img(off, 0);
+  for (int s = 0; s < j->nslices; ++s) {
+    const int n = j->slice_n[s];
+    for (int t = 0; t < j->T; ++t)
+      std::copy(codes + (size_t)t * N + (size_t)s * j->slice,
+                codes + (size_t)t * N + (size_t)s * j->slice + n, img.data() + off_codes[s] + (size_t)t * n);
+    std::copy(reinterpret_cast<const uint8_t*>(weights + (size_t)s * j->slice),
+              reinterpret_cast<const uint8_t*>(weights + (size_t)s * j->slice + n),
+              img.data() + off_w[s]);
+  }
+  for (int q = 0; q < ncodes * S; ++q) {
+    if (dtype == HX_F64) reinterpret_cast<double*>(img.data() + o_table)[q] = table[q];
+    else reinterpret_cast<float*>(img.data() + o_table)[q] = (float)table[q];
+  }
+  std::copy(lambda, lambda + S, reinterpret_cast<double*>(img.data() + o_lam));
+  std::copy(left, left + S * S, reinterpret_cast<double*>(img.data() + o_left));
+  std::copy(right, right + S * S, reinterpret_cast<double*>(img.data() + o_right));
+  std::copy(rates, rates + C, reinterpret_cast<double*>(img.data() + o_rates));
+  for (int q = 0; q < C; ++q) {
+    if (dtype == HX_F64) reinterpret_cast<double*>(img.data() + o_props)[q] = props[q];
+    else reinterpret_cast<float*>(img.data() + o_props)[q] = (float)props[q];
+  }
+  for (int q = 0; q < S; ++q) {
+    if (dtype == HX_F64) reinterpret_cast<double*>(img.data() + o_freqs)[q] = freqs[q];
+    else reinterpret_cast<float*>(img.data() + o_freqs)[q] = (float)freqs[q];
+  }
+  int prev = 0;
+  hipError_t e = hipGetDevice(&prev);
+  if (e == hipSuccess) e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipMalloc(&j->d_buf, off);
+  if (e == hipSuccess) e = hipMemcpy(j->d_buf, img.data(), off, hipMemcpyHostToDevice);
+  hipSetDevice(prev);
+  if (e != hipSuccess) {
+    hx::set_error(std::string("job upload: ") + hipGetErrorString(e));
+    if (j->d_buf) hipFree(j->d_buf);
+    delete j;
+    return e == hipErrorOutOfMemory ? HX_ERR_OUT_OF_MEMORY : HX_ERR_HIP;
+  }
+  char* base = static_cast<char*>(j->d_buf);
+  for (int s = 0; s < j->nslices; ++s) {
+    j->d_codes.push_back(reinterpret_cast<uint8_t*>(base + off_codes[s]));
+    j->d_weights.push_back(reinterpret_cast<double*>(base + off_w[s]));
+  }
+  j->d_table = base + o_table;
+  j->d_lambda = reinterpret_cast<double*>(base + o_lam);
+  j->d_left = reinterpret_cast<double*>(base + o_left);
+  j->d_right = reinterpret_cast<double*>(base + o_right);
+  j->d_rates = reinterpret_cast<double*>(base + o_rates);
+  j->d_props = base + o_props;
+  j->d_freqs = base + o_freqs;
+  j->resident = off;
+  *out = j;
+  return HX_OK;
+}
+
+int hx_job_destroy(hx_job_t job) {
+  if (!job) return HX_OK;
+  if (job->d_buf) hipFree(job->d_buf);
+  delete job;
+  return HX_OK;
+}
+
+size_t hx_job_resident_bytes(hx_job_t job) { return job ? job->resident : 0; }
+
+size_t hx_job_workspace_bytes(hx_job_t job, int K, int want_grad) {
+  if (!job || K < 1) return 0;
+  return hx::job_ws(job, K, want_grad).total;
+}
+
+int hx_job_eval(hx_job_t job, int K, const double* bls, double* out_loglik, double* out_dbls,
+                void* workspace, size_t workspace_bytes, hx_stream_t stream) {
+  HX_REQUIRE(job && bls && out_loglik && workspace && K >= 1, HX_ERR_INVALID_ARGUMENT,
+             "null argument");
+  const int grad = out_dbls != nullptr;
+  const hx::JobWs w = hx::job_ws(job, K, grad);
+  HX_REQUIRE(workspace_bytes >= w.total, HX_ERR_WORKSPACE_TOO_SMALL, "workspace too small");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  char* base = static_cast<char*>(workspace);
+  void* mats = base + w.mats;
+  void* mats_dt = grad ? base + w.mats_dt : nullptr;
+  int rc = hx_pmatrix(job->S, K, job->B, job->C, 0, job->d_lambda, job->d_left, job->d_right, bls,
+                      job->d_rates, 0, mats, job->dtype, stream);
+  if (rc != HX_OK) return rc;
+  if (grad) {
+    rc = hx_pmatrix_dtau(job->S, K, job->B, job->C, 0, job->d_lambda, job->d_left, job->d_right,
+                         bls, job->d_rates, 0, mats_dt, job->dtype, stream);
+    if (rc != HX_OK) return rc;
+  }
+  double* ll_tmp = reinterpret_cast<double*>(base + w.loglik_tmp);
+  const size_t nt = (size_t)K * job->B * job->C;
+  double* dt_tmp = grad ? reinterpret_cast<double*>(base + w.dtau_tmp) : nullptr;
+  double* dt_acc = grad ? reinterpret_cast<double*>(base + w.dtau_acc) : nullptr;
+  hx::fill0(out_loglik, K, st);
+  if (grad) hx::fill0(dt_acc, nt, st);
+  for (int s = 0; s < job->nslices; ++s) {
+    const int n = job->slice_n[s];
+    rc = hx_loglik(job->plan, job->S, n, job->C, K, job->dtype, HX_TIPS_CODES, job->d_codes[s],
+                   job->d_table, job->ncodes, job->d_weights[s], mats, job->d_freqs, 0,
+                   job->d_props, 0, base + w.prune, w.prune_bytes, ll_tmp, nullptr, nullptr,
+                   stream);
+    if (rc != HX_OK) return rc;
+    hx::accum(out_loglik, ll_tmp, K, st);
+    if (grad) {
+      rc = hx_loglik_grad_tau(job->plan, job->S, n, job->C, K, job->dtype, HX_TIPS_CODES,
+                              job->d_codes[s], job->d_table, job->ncodes, job->d_weights[s], mats,
+                              mats_dt, job->d_freqs, 0, job->d_props, 0, base + w.prune,
+                              w.prune_bytes, dt_tmp, stream);
+      if (rc != HX_OK) return rc;
+      hx::accum(dt_acc, dt_tmp, nt, st);
+    }
+  }
+  if (grad) {
+    const size_t nb = (size_t)K * job->B;
+    hx::k_dbls<<<(unsigned)((nb + 255) / 256), 256, 0, st>>>(K, job->B, job->C, job->d_rates,
+                                                             dt_acc, out_dbls);
+  }
+  HX_HIP_TRY(hipGetLastError());
+  return HX_OK;
+}
+
+}  // extern "C"

@@ -20,7 +20,7 @@ template <typename O>
 __global__ void k_pmatrix_elem(int S, int K, int B, int C, int eb, const double* __restrict__ lam,
                                const double* __restrict__ L, const double* __restrict__ R,
                                const double* __restrict__ bls, const double* __restrict__ rates,
-                               int rb, O* __restrict__ out) {
+                               int rb, O* __restrict__ out, int order) {
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t SS = (size_t)S * S;
   const size_t total = (size_t)K * B * C * SS;
@@ -37,7 +37,10 @@ __global__ void k_pmatrix_elem(int S, int K, int B, int C, int eb, const double*
   const double* re = R + (size_t)e * SS;
   const double* la = lam + (size_t)e * S;
   double acc = 0.0;
-  for (int m = 0; m < S; ++m) acc = fma(le[i * S + m] * exp(la[m] * tau), re[m * S + j], acc);
+  for (int m = 0; m < S; ++m) {
+    const double d = order ? la[m] * exp(la[m] * tau) : exp(la[m] * tau);
+    acc = fma(le[i * S + m] * d, re[m * S + j], acc);
+  }
   out[idx] = (O)acc;
 }
 
@@ -46,7 +49,7 @@ template <typename O>
 __global__ void k_pmatrix_block(int S, int K, int B, int C, int eb, const double* __restrict__ lam,
                                 const double* __restrict__ L, const double* __restrict__ R,
                                 const double* __restrict__ bls, const double* __restrict__ rates,
-                                int rb, O* __restrict__ out) {
+                                int rb, O* __restrict__ out, int order) {
   __shared__ double ex[64];
   const size_t mat = blockIdx.x;
   const int c = (int)(mat % C);
@@ -55,7 +58,10 @@ __global__ void k_pmatrix_block(int S, int K, int B, int C, int eb, const double
   const int e = eb ? k : 0;
   const size_t SS = (size_t)S * S;
   const double tau = bls[(size_t)k * B + b] * rates[(size_t)(rb ? k : 0) * C + c];
-  if (threadIdx.x < S) ex[threadIdx.x] = exp(lam[(size_t)e * S + threadIdx.x] * tau);
+  if (threadIdx.x < S) {
+    const double lm = lam[(size_t)e * S + threadIdx.x];
+    ex[threadIdx.x] = order ? lm * exp(lm * tau) : exp(lm * tau);
+  }
   __syncthreads();
   const double* le = L + (size_t)e * SS;
   const double* re = R + (size_t)e * SS;
@@ -252,9 +258,10 @@ __global__ void k_pgrad_q(int S, const double* __restrict__ L, const double* __r
 
 extern "C" {
 
-int hx_pmatrix(int S, int K, int B, int C, int eigen_batched, const double* lambda,
-               const double* left, const double* right, const double* bls, const double* rates,
-               int rates_batched, void* mats, int dtype, hx_stream_t stream) {
+static int pmatrix_impl(int S, int K, int B, int C, int eigen_batched, const double* lambda,
+                        const double* left, const double* right, const double* bls,
+                        const double* rates, int rates_batched, void* mats, int dtype,
+                        hx_stream_t stream, int order) {
   HX_REQUIRE(S >= 2 && S <= 64 && K >= 1 && B >= 1 && C >= 1, HX_ERR_INVALID_ARGUMENT,
              "bad dimensions (S must be 2..64)");
   HX_REQUIRE(lambda && left && right && bls && rates && mats, HX_ERR_INVALID_ARGUMENT,
@@ -268,26 +275,39 @@ int hx_pmatrix(int S, int K, int B, int C, int eigen_batched, const double* lamb
     if (dtype == HX_F64)
       hx::k_pmatrix_elem<double><<<grid, 256, 0, st>>>(S, K, B, C, eigen_batched, lambda, left,
                                                         right, bls, rates, rates_batched,
-                                                        static_cast<double*>(mats));
+                                                        static_cast<double*>(mats), order);
     else
       hx::k_pmatrix_elem<float><<<grid, 256, 0, st>>>(S, K, B, C, eigen_batched, lambda, left,
                                                        right, bls, rates, rates_batched,
-                                                       static_cast<float*>(mats));
+                                                       static_cast<float*>(mats), order);
   } else {
     HX_REQUIRE(nmat < (1u << 31), HX_ERR_UNSUPPORTED, "too many matrices");
     if (dtype == HX_F64)
-      hx::k_pmatrix_block<double><<<(unsigned)nmat, 256, 0, st>>>(S, K, B, C, eigen_batched, lambda,
-                                                                  left, right, bls, rates,
-                                                                  rates_batched,
-                                                                  static_cast<double*>(mats));
+      hx::k_pmatrix_block<double><<<(unsigned)nmat, 256, 0, st>>>(
+          S, K, B, C, eigen_batched, lambda, left, right, bls, rates, rates_batched,
+          static_cast<double*>(mats), order);
     else
-      hx::k_pmatrix_block<float><<<(unsigned)nmat, 256, 0, st>>>(S, K, B, C, eigen_batched, lambda,
-                                                                 left, right, bls, rates,
-                                                                 rates_batched,
-                                                                 static_cast<float*>(mats));
+      hx::k_pmatrix_block<float><<<(unsigned)nmat, 256, 0, st>>>(
+          S, K, B, C, eigen_batched, lambda, left, right, bls, rates, rates_batched,
+          static_cast<float*>(mats), order);
   }
   HX_HIP_TRY(hipGetLastError());
   return HX_OK;
+}
+
+int hx_pmatrix(int S, int K, int B, int C, int eigen_batched, const double* lambda,
+               const double* left, const double* right, const double* bls, const double* rates,
+               int rates_batched, void* mats, int dtype, hx_stream_t stream) {
+  return pmatrix_impl(S, K, B, C, eigen_batched, lambda, left, right, bls, rates, rates_batched,
+                      mats, dtype, stream, 0);
+}
+
+int hx_pmatrix_dtau(int S, int K, int B, int C, int eigen_batched, const double* lambda,
+                    const double* left, const double* right, const double* bls,
+                    const double* rates, int rates_batched, void* mats_dtau, int dtype,
+                    hx_stream_t stream) {
+  return pmatrix_impl(S, K, B, C, eigen_batched, lambda, left, right, bls, rates, rates_batched,
+                      mats_dtau, dtype, stream, 1);
 }
 
 static size_t pgrad_layout(int S, int K, int B, int C, size_t* o_xpart, size_t* o_x) {

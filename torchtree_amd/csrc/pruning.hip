@@ -14,7 +14,9 @@
 //   scales   [slot][K][N] int32   cumulative power-of-two exponent of the
 //                                 subtree below the node
 //   rootsums [K][chunk][1+S+C]    per-chunk FP64 partial sums at the root
-//   gpart    [B][K][chunk][C][S][S]  per-chunk partial d loglik / d mats
+//   gpart    [B][K][chunk][C][S][S]  per-chunk partial d loglik / d mats, or
+//   gtau     [B][K][chunk][C]        per-chunk d loglik / d tau (tau = t_b r_c),
+//                                    the contraction of the former with dP/dtau
 //
 // Rescaling: every node's (C,S)-maximum per (sample, pattern) is brought into
 // [0.5, 1) by an exact power of two (frexp/ldexp), the exponent is added to
@@ -353,7 +355,21 @@ __global__ __launch_bounds__(kBlock) void k_pre(PassArgs<R> a, const int32_t* __
     g[S * S + i] = (double)gb[i];
   }
   block_sum_by_c<CP, NG>(g, red);
-  if (threadIdx.x < a.C) {
+  if (a.mats_dt != nullptr) {
+    // d loglik / d tau_{b,c} = sum_ij G_bc[i][j] dP_bc/dtau[i][j]
+    if (threadIdx.x < a.C) {
+      const R* qa = a.mats_dt + (((size_t)k * a.B + ca) * a.C + threadIdx.x) * S * S;
+      const R* qb = a.mats_dt + (((size_t)k * a.B + cb) * a.C + threadIdx.x) * S * S;
+      double sa = 0.0, sb = 0.0;
+#pragma unroll
+      for (int i = 0; i < S * S; ++i) {
+        sa = fma(g[i], (double)qa[i], sa);
+        sb = fma(g[S * S + i], (double)qb[i], sb);
+      }
+      a.gtau[(((size_t)ca * a.K + k) * a.nchunks + chunk) * a.C + threadIdx.x] = sa;
+      a.gtau[(((size_t)cb * a.K + k) * a.nchunks + chunk) * a.C + threadIdx.x] = sb;
+    }
+  } else if (threadIdx.x < a.C) {
     const size_t css = (size_t)a.C * S * S;
     R* oa = a.gpart + (((size_t)ca * a.K + k) * a.nchunks + chunk) * css + (size_t)threadIdx.x * S * S;
     R* ob = a.gpart + (((size_t)cb * a.K + k) * a.nchunks + chunk) * css + (size_t)threadIdx.x * S * S;
@@ -398,12 +414,27 @@ __global__ void k_reduce_g(const R* __restrict__ gpart, double* __restrict__ dma
   dmats[idx] = t;
 }
 
+__global__ void k_reduce_tau(const double* __restrict__ gtau, double* __restrict__ dtau, int K,
+                             int B, int nchunks, int C) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)K * B * C) return;
+  const int c = (int)(idx % C);
+  const size_t kb = idx / C;
+  const int b = (int)(kb % B), k = (int)(kb / B);
+  const double* src = gtau + (((size_t)b * K + k) * nchunks) * C + c;
+  double t = 0.0;
+  for (int ch = 0; ch < nchunks; ++ch) t += src[(size_t)ch * C];
+  dtau[idx] = t;
+}
+
 // ------------------------------------------------------------------- host side
 struct WsLayout {
   size_t partials, scales, rootsums, gpart, total;
 };
 
-static WsLayout ws_layout(const hx_plan_s* plan, int S, int N, int C, int K, int dtype) {
+// grad_mode: 0 loglik only, 1 d loglik / d mats, 2 d loglik / d tau.
+static WsLayout ws_layout(const hx_plan_s* plan, int S, int N, int C, int K, int dtype,
+                          int grad_mode) {
   const size_t rb = dtype == HX_F64 ? 8 : 4;
   const Tiling t = make_tiling(N, C, K, S, (int)rb);
   const size_t I = plan->I, B = 2 * (size_t)plan->T - 2;
@@ -416,7 +447,8 @@ static WsLayout ws_layout(const hx_plan_s* plan, int S, int N, int C, int K, int
   w.rootsums = off;
   off = align_up(off + (size_t)K * t.nchunks * (1 + S + C) * 8, 256);
   w.gpart = off;
-  off = align_up(off + B * K * (size_t)t.nchunks * C * S * S * rb, 256);
+  if (grad_mode == 1) off = align_up(off + B * K * (size_t)t.nchunks * C * S * S * rb, 256);
+  if (grad_mode == 2) off = align_up(off + B * K * (size_t)t.nchunks * C * 8, 256);
   w.total = off;
   return w;
 }
@@ -484,6 +516,8 @@ struct CallArgs {
   int props_batched;
   void* ws;
   size_t ws_bytes;
+  int grad_mode;
+  const void* mats_dt;
 };
 
 static int validate(const CallArgs& c) {
@@ -504,14 +538,14 @@ static int validate(const CallArgs& c) {
                "tip table must have 1..256 rows");
   HX_REQUIRE(c.tips && c.weights && c.mats && c.freqs && c.props && c.ws, HX_ERR_INVALID_ARGUMENT,
              "null pointer argument");
-  const WsLayout w = ws_layout(c.plan, c.S, c.N, c.C, c.K, c.dtype);
+  const WsLayout w = ws_layout(c.plan, c.S, c.N, c.C, c.K, c.dtype, c.grad_mode);
   HX_REQUIRE(c.ws_bytes >= w.total, HX_ERR_WORKSPACE_TOO_SMALL, "workspace too small");
   return HX_OK;
 }
 
 template <typename R>
 static PassArgs<R> make_args(const CallArgs& c) {
-  const WsLayout w = ws_layout(c.plan, c.S, c.N, c.C, c.K, c.dtype);
+  const WsLayout w = ws_layout(c.plan, c.S, c.N, c.C, c.K, c.dtype, c.grad_mode);
   const Tiling t = make_tiling(c.N, c.C, c.K, c.S, c.dtype == HX_F64 ? 8 : 4);
   char* base = static_cast<char*>(c.ws);
   PassArgs<R> a;
@@ -538,6 +572,8 @@ static PassArgs<R> make_args(const CallArgs& c) {
   a.weights = c.weights;
   a.rootsums = reinterpret_cast<double*>(base + w.rootsums);
   a.gpart = reinterpret_cast<R*>(base + w.gpart);
+  a.mats_dt = c.grad_mode == 2 ? static_cast<const R*>(c.mats_dt) : nullptr;
+  a.gtau = reinterpret_cast<double*>(base + w.gpart);
   return a;
 }
 
@@ -562,7 +598,7 @@ static int run_post(const CallArgs& c, double* out_l, double* out_f, double* out
 }
 
 template <typename R>
-static int run_pre(const CallArgs& c, double* out_dmats, hipStream_t st) {
+static int run_pre(const CallArgs& c, double* out, hipStream_t st) {
   const PassArgs<R> a = make_args<R>(c);
   const hx_plan_s* plan = c.plan;
   const int D = (int)plan->pre_off.size() - 1;
@@ -573,10 +609,16 @@ static int run_pre(const CallArgs& c, double* out_dmats, hipStream_t st) {
                  : launch_pre_wide<R>(a, plan->d_pre + off, width, c.tip_kind, d == 0, c.S, st);
     if (rc != HX_OK) return rc;
   }
-  const int CSS = c.C * c.S * c.S;
-  const size_t total = (size_t)c.K * a.B * CSS;
-  k_reduce_g<R><<<(unsigned)((total + 255) / 256), 256, 0, st>>>(a.gpart, out_dmats, c.K, a.B,
-                                                                 a.nchunks, CSS);
+  if (c.grad_mode == 2) {
+    const size_t total = (size_t)c.K * a.B * c.C;
+    k_reduce_tau<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(a.gtau, out, c.K, a.B, a.nchunks,
+                                                                  c.C);
+  } else {
+    const int CSS = c.C * c.S * c.S;
+    const size_t total = (size_t)c.K * a.B * CSS;
+    k_reduce_g<R><<<(unsigned)((total + 255) / 256), 256, 0, st>>>(a.gpart, out, c.K, a.B,
+                                                                   a.nchunks, CSS);
+  }
   HX_HIP_TRY(hipGetLastError());
   return HX_OK;
 }
@@ -585,9 +627,9 @@ static int run_pre(const CallArgs& c, double* out_dmats, hipStream_t st) {
 
 extern "C" {
 
-size_t hx_workspace_bytes(hx_plan_t plan, int S, int N, int C, int K, int dtype) {
-  if (!plan) return 0;
-  return hx::ws_layout(plan, S, N, C, K, dtype).total;
+size_t hx_workspace_bytes(hx_plan_t plan, int S, int N, int C, int K, int dtype, int grad_mode) {
+  if (!plan || grad_mode < 0 || grad_mode > 2) return 0;
+  return hx::ws_layout(plan, S, N, C, K, dtype, grad_mode).total;
 }
 
 int hx_loglik(hx_plan_t plan, int S, int N, int C, int K, int dtype, int tip_kind, const void* tips,
@@ -596,7 +638,7 @@ int hx_loglik(hx_plan_t plan, int S, int N, int C, int K, int dtype, int tip_kin
               void* workspace, size_t workspace_bytes, double* out_loglik, double* out_dfreqs,
               double* out_dprops, hx_stream_t stream) {
   hx::CallArgs c{plan, S, N, C, K, dtype, tip_kind, tips, tip_table, ncodes, weights, mats,
-                 freqs, freqs_batched, props, props_batched, workspace, workspace_bytes};
+                 freqs, freqs_batched, props, props_batched, workspace, workspace_bytes, 0, nullptr};
   int rc = hx::validate(c);
   if (rc != HX_OK) return rc;
   HX_REQUIRE(out_loglik != nullptr, HX_ERR_INVALID_ARGUMENT, "null output");
@@ -611,13 +653,30 @@ int hx_loglik_grad_mats(hx_plan_t plan, int S, int N, int C, int K, int dtype, i
                         int props_batched, void* workspace, size_t workspace_bytes,
                         double* out_dmats, hx_stream_t stream) {
   hx::CallArgs c{plan, S, N, C, K, dtype, tip_kind, tips, tip_table, ncodes, weights, mats,
-                 freqs, freqs_batched, props, props_batched, workspace, workspace_bytes};
+                 freqs, freqs_batched, props, props_batched, workspace, workspace_bytes, 1, nullptr};
   int rc = hx::validate(c);
   if (rc != HX_OK) return rc;
   HX_REQUIRE(out_dmats != nullptr, HX_ERR_INVALID_ARGUMENT, "null output");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   if (dtype == HX_F64) return hx::run_pre<double>(c, out_dmats, st);
   return hx::run_pre<float>(c, out_dmats, st);
+}
+
+int hx_loglik_grad_tau(hx_plan_t plan, int S, int N, int C, int K, int dtype, int tip_kind,
+                       const void* tips, const void* tip_table, int ncodes, const double* weights,
+                       const void* mats, const void* mats_dtau, const void* freqs,
+                       int freqs_batched, const void* props, int props_batched, void* workspace,
+                       size_t workspace_bytes, double* out_dtau, hx_stream_t stream) {
+  hx::CallArgs c{plan, S, N, C, K, dtype, tip_kind, tips, tip_table, ncodes, weights, mats,
+                 freqs, freqs_batched, props, props_batched, workspace, workspace_bytes, 2,
+                 mats_dtau};
+  int rc = hx::validate(c);
+  if (rc != HX_OK) return rc;
+  HX_REQUIRE(out_dtau != nullptr && mats_dtau != nullptr, HX_ERR_INVALID_ARGUMENT,
+             "null output or dP/dtau");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (dtype == HX_F64) return hx::run_pre<double>(c, out_dtau, st);
+  return hx::run_pre<float>(c, out_dtau, st);
 }
 
 }  // extern "C"

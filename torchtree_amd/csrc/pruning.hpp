@@ -23,7 +23,9 @@ struct PassArgs {
   int props_ks;           // stride between samples (0 or C)
   const double* weights;  // [N]
   double* rootsums;       // [K][nchunks][1+S+C]
-  R* gpart;               // [B][K][nchunks][C][S][S]
+  R* gpart;               // [B][K][nchunks][C][S][S]     (d loglik / d mats mode)
+  const R* mats_dt;       // [K][B][C][S][S] dP/dtau; non-null selects d loglik / d tau mode
+  double* gtau;           // [B][K][nchunks][C]            (d loglik / d tau mode)
 };
 
 constexpr double kLn2 = 0.69314718055994530942;

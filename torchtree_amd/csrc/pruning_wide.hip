@@ -339,7 +339,18 @@ __global__ __launch_bounds__(kWideMaxThreads) void k_pre_wide(PassArgs<R> a,
     }
     __syncthreads();
   }
-  if (i < S) {
+  if (a.mats_dt != nullptr) {
+    // d loglik / d tau_{b,c}: row i of G_h^c against row i of dP/dtau, summed over i
+    double s = 0.0;
+    if (i < S) {
+      const R* qrow = a.mats_dt + (((size_t)k * a.B + chh) * C + c) * S * S + (size_t)i * S;
+#pragma unroll
+      for (int j = 0; j < SP; ++j)
+        if (j < S) s = fma((double)g[j], (double)qrow[j], s);
+    }
+    s = group_sum<SP>(s);
+    if (i == 0) a.gtau[(((size_t)chh * a.K + k) * a.nchunks + chunk) * C + c] = s;
+  } else if (i < S) {
     const size_t css = (size_t)C * S * S;
     R* o = a.gpart + (((size_t)chh * a.K + k) * a.nchunks + chunk) * css + ((size_t)c * S + i) * S;
 #pragma unroll

@@ -119,4 +119,4 @@ class TreeJob:
         from ._lib import HX_F32, HX_F64
 
         code = HX_F64 if self.dtype == F64 else HX_F32
-        return self.plan.workspace_bytes(self.S, self.N, C, K, code)
+        return self.plan.workspace_bytes(self.S, self.N, C, K, code, _lib.HX_GRAD_MATS)

@@ -61,8 +61,8 @@ class Plan:
         _lib.check(lib.hx_plan_info(handle, ctypes.byref(pl), ctypes.byref(dl), ctypes.byref(mw)))
         self.post_levels, self.pre_levels, self.max_width = pl.value, dl.value, mw.value
 
-    def workspace_bytes(self, S, N, C, K, dtype_code) -> int:
-        return int(_lib.load().hx_workspace_bytes(self.handle, S, N, C, K, dtype_code))
+    def workspace_bytes(self, S, N, C, K, dtype_code, grad_mode=_lib.HX_GRAD_MATS) -> int:
+        return int(_lib.load().hx_workspace_bytes(self.handle, S, N, C, K, dtype_code, grad_mode))
 
     def __del__(self):
         h = getattr(self, "handle", None)
@@ -201,7 +201,8 @@ class Pruning:
         dtype = mats.dtype
         K, B, C, S, code, tips_p, table_p = self._common(mats, freqs, props, dtype)
         dev = mats.device
-        nbytes = self.plan.workspace_bytes(S, self.N, C, K, code)
+        mode = _lib.HX_GRAD_MATS if want_root_grads else _lib.HX_GRAD_NONE
+        nbytes = self.plan.workspace_bytes(S, self.N, C, K, code, mode)
         ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
         out = torch.empty(K, dtype=torch.float64, device=dev)
         dfreqs = torch.empty(K, S, dtype=torch.float64, device=dev) if want_root_grads else None

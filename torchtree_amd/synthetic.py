@@ -9,6 +9,7 @@ post-order with the root last, branch b = node b. Unrooted trees carry
 from __future__ import annotations
 
 import dataclasses
+import math
 
 import numpy as np
 import torch
@@ -291,3 +292,73 @@ def config3(seed=3, N=50_000, K=16, T=1024):
     """GTR+Weibull4, 1,024 taxa, 50,000 patterns, K=16."""
     return make_workload(T, N, K, model="gtr", categories=4, site="weibull", shape=0.5,
                          bl_mean=0.02, seed=seed, name=f"config3-gtr-w4-{T}taxa-{N}pat-K{K}")
+
+
+# ------------------------------------------------------------ serving mix
+
+
+def random_job_spec(kind, T, N, rng, name=""):
+    """A resident-job spec with random tip states (2% unknown) for serving.
+
+    kind: "dna" (GTR + Weibull4(0.5)), "aa" (20 states, Gamma(1,1)
+    exchangeabilities, Dirichlet(1) frequencies, gamma4(0.5)), "codon"
+    (MG94 kappa=2, omega=0.3, uniform frequencies, gamma4(0.5)).
+    """
+    topo = random_topology(T, rng)
+    if kind == "dna":
+        S = 4
+        freqs = rng.dirichlet(np.full(4, 10.0))
+        q = models.gtr_q(torch.tensor(rng.gamma(2.0, 2.0, 6)), torch.tensor(freqs))
+        rates, props = models.weibull_rates(torch.tensor([0.5], dtype=torch.float64), 4)
+    elif kind == "aa":
+        S = 20
+        freqs = rng.dirichlet(np.ones(20))
+        q = models.q_from_exchangeabilities(torch.tensor(rng.gamma(1.0, 1.0, 190)), torch.tensor(freqs))
+        rates, props = models.gamma_rates(0.5, 4)
+    elif kind == "codon":
+        S = 61
+        freqs = np.full(61, 1.0 / 61)
+        one = torch.ones(1, dtype=torch.float64)
+        q = models.mg94_q(one, 0.3 * one, 2.0 * one, torch.tensor(freqs))
+        rates, props = models.gamma_rates(0.5, 4)
+    else:
+        raise ValueError(kind)
+    q_norm = models.normalize(q, torch.tensor(freqs))
+    codes = rng.integers(0, S, size=(T, N), dtype=np.uint8)
+    unknown = rng.random((T, N)) < 0.02
+    codes[unknown] = S
+    table = np.concatenate((np.eye(S), np.ones((1, S))))
+    bl = unrooted_lengths(topo, rng, 0.05)
+    return dict(name=name, kind=kind, topo=topo, S=S, codes=codes, table=table,
+                weights=rng.poisson(3.0, N) + 1.0, q_norm=q_norm, freqs=freqs,
+                rates=rates.reshape(-1).numpy(), props=props.reshape(-1).numpy(), bl=bl)
+
+
+def serving_mix(n_jobs=200, seed=5, t_range=(16, 1024), n_range=(100, 100_000),
+                mix=(("dna", 0.80), ("aa", 0.15), ("codon", 0.05))):
+    """Job specs of the serving configuration: T ~ U{16..1024}, N log-uniform, 80/15/5 types."""
+    rng = np.random.default_rng(seed)
+    kinds, probs = zip(*mix)
+    specs = []
+    for j in range(n_jobs):
+        kind = kinds[rng.choice(len(kinds), p=probs)]
+        T = int(rng.integers(t_range[0], t_range[1] + 1))
+        N = int(math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1]))))
+        specs.append(random_job_spec(kind, T, N, rng, name=f"job{j}-{kind}-T{T}-N{N}"))
+    return specs
+
+
+def request_stream(n_requests, n_jobs, rate, seed=6, ks=(1, 8, 32), sigma=0.1):
+    """(job index, K, log-normal jitter seed, arrival time) of Poisson arrivals at ``rate``/s."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    return [(int(rng.integers(n_jobs)), int(rng.choice(ks)), int(rng.integers(1 << 30)), float(t[i]))
+            for i in range(n_requests)]
+
+
+def draw_bls(spec, K, seed, sigma=0.1):
+    rng = np.random.default_rng(seed)
+    bl = spec["bl"]
+    draws = bl[None, :] * np.exp(sigma * rng.standard_normal((K, bl.shape[0])))
+    draws[:, -1] = 0.0
+    return torch.tensor(draws, dtype=torch.float64)
