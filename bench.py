@@ -49,6 +49,14 @@ def parse():
     ap.add_argument("--cpu-patterns", type=int, default=10000)
     ap.add_argument("--cpu-samples", type=int, default=2)
     ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--config", default="c2", choices=("c1", "c2", "c3", "c4a", "c4b"),
+                    help="workload of the timed step (c2 is the headline configuration)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-mix measurement")
+    ap.add_argument("--serve-jobs", type=int, default=200)
+    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-plain-limit", type=float, default=60.0,
+                    help="seconds of sequential plain FP64 evaluation (stats over completed requests)")
+    ap.add_argument("--serve-streams", type=int, default=4)
     return ap.parse_args()
 
 
@@ -85,12 +93,21 @@ def main():
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     t0 = time.time()
-    wl = synthetic.config2(seed=args.seed, N=args.patterns, K=args.samples, T=args.taxa)
-    log(f"workload {wl.name}: T={wl.T} N={wl.N} C={wl.C} K={wl.K} built in {time.time()-t0:.1f}s")
+    wl = make_config(args, synthetic)
+    log(f"workload {wl.name}: T={wl.T} N={wl.N} C={wl.C} K={wl.K} S={wl.S} built in {time.time()-t0:.1f}s")
     post = [tuple(int(x) for x in r) for r in wl.topo.postorder]
-    tips = TipData.from_states(list(wl.codes), 4, dev)
+    S = wl.S
+    tips = TipData.from_states(list(wl.codes), S, dev)
     rates = wl.rates.reshape(1, -1).to(dev)
     props = wl.props.reshape(1, -1).to(dev)
+
+    q_builder = {4: hm.gtr_q, 20: hm.q_from_exchangeabilities}.get(S)
+    if S == 61:
+        flags = hm.mg94_flags()
+        one = torch.ones(1, dtype=torch.float64)
+
+        def q_builder(ex, fr):  # exch holds (alpha, beta=omega, kappa)
+            return hm.mg94_q(ex[0:1], ex[1:2], ex[2:3], fr, flags)
 
     def make_step(job):
         bls = wl.bls.to(dev).clone().requires_grad_(True)
@@ -101,7 +118,7 @@ def main():
             bls.grad = None
             exch.grad = None
             freqs.grad = None
-            qn = hm.normalize(hm.gtr_q(exch, freqs), freqs)
+            qn = hm.normalize(q_builder(exch, freqs), freqs)
             ll = job.loglik(bls, rates, props, qn, freqs)
             ll.sum().backward()
             return ll, bls, exch, freqs
@@ -123,7 +140,7 @@ def main():
         return total, lat
 
     # ---- FP32 served-precision path (the measured value)
-    job32 = TreeJob(post, tips, wl.weights, 4, dtype=torch.float32, device=dev)
+    job32 = TreeJob(post, tips, wl.weights, S, dtype=torch.float32, device=dev)
     step32 = make_step(job32)
     total, lat = timed(step32, args.steps, args.warmup)
     ms = total / args.steps * 1e3
@@ -137,7 +154,7 @@ def main():
     ll32 = ll32.detach().cpu()
     g32 = (b32.grad.detach().cpu(), e32.grad.detach().cpu(), f32.grad.detach().cpu())
     nchunks = _nchunks(wl.N, wl.C, wl.K)
-    post_b, pre_b = algorithmic_bytes(wl.T, wl.N, wl.C, wl.K, 4, 4, nchunks)
+    post_b, pre_b = algorithmic_bytes(wl.T, wl.N, wl.C, wl.K, S, 4, nchunks)
     post_ms = phases["postorder"]["ms"]
     pre_ms = phases["preorder"]["ms"]
     post_gbs = post_b / (post_ms * 1e-3) / 1e9
@@ -162,9 +179,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "batched DNA VI loglik+grad: GTR+Weibull4(0.5), random "
-                        f"{wl.T}-taxon tree, {wl.N} unique patterns, K={wl.K} draws",
-            "taxa": wl.T, "patterns": wl.N, "categories": wl.C, "samples": wl.K, "states": 4,
+            "workload": DESCRIPTIONS[args.config].format(T=wl.T, N=wl.N, K=wl.K),
+            "taxa": wl.T, "patterns": wl.N, "categories": wl.C, "samples": wl.K, "states": S,
             "partials": "fp32 + power-of-two rescaling", "seed": args.seed,
         },
         "p50_ms": statistics.median(lat),
@@ -184,7 +200,7 @@ def main():
 
     # ---- FP64 plain path: accuracy reference at full size, and its throughput
     if not args.no_check:
-        job64 = TreeJob(post, tips, wl.weights, 4, dtype=torch.float64, device=dev)
+        job64 = TreeJob(post, tips, wl.weights, S, dtype=torch.float64, device=dev)
         step64 = make_step(job64)
         if not args.no_fp64_timing:
             tot64, lat64 = timed(step64, max(2, args.steps // 3), 1)
@@ -200,9 +216,113 @@ def main():
         torch.cuda.empty_cache()
         log(f"fp32 vs fp64: {result['logl_rel_err_vs_fp64']:.2e} {result['grad_rel_err_vs_fp64']}")
 
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.config == "c2":
         result["cpu_baseline"] = cpu_baseline(args, dev)
+    if not args.no_serve and args.config == "c2":
+        result["serving"] = serving_bench(args, dev)
     print(json.dumps(result), flush=True)
+
+
+DESCRIPTIONS = {
+    "c1": "plain JC69 loglik+grad: random {T}-taxon tree, {N} patterns from 1,000 simulated sites, K={K}",
+    "c2": "batched DNA VI loglik+grad: GTR+Weibull4(0.5), random {T}-taxon tree, {N} unique patterns, K={K} draws",
+    "c3": "adjoint DNA loglik+grad: GTR+Weibull4(0.5), random {T}-taxon tree, {N} patterns, K={K}",
+    "c4a": "amino acid loglik+grad: 20 states, Gamma(1,1) exchangeabilities, gamma4, {T} taxa, {N} patterns, K={K}",
+    "c4b": "codon loglik+grad: MG94 kappa=2 omega=0.3, gamma4, {T} taxa, {N} patterns, K={K}",
+}
+
+
+def make_config(args, synthetic):
+    if args.config == "c1":
+        return synthetic.config1()
+    if args.config == "c2":
+        return synthetic.config2(seed=args.seed, N=args.patterns, K=args.samples, T=args.taxa)
+    if args.config == "c3":
+        return synthetic.config3()
+    if args.config == "c4a":
+        return synthetic.make_workload(256, 20_000, 1, model="aa", site="gamma", shape=0.5,
+                                       bl_mean=0.05, seed=41, name="config4a-aa-256taxa-20000pat")
+    return synthetic.make_workload(128, 10_000, 1, model="mg94", site="gamma", shape=0.5,
+                                   bl_mean=0.05, seed=42, name="config4b-mg94-128taxa-10000pat")
+
+
+def serving_bench(args, dev):
+    """Serving mix: resident jobs, Poisson loglik+grad requests, FP32 served vs plain FP64."""
+    import torch
+
+    from torchtree_amd import serving, synthetic
+
+    t0 = time.time()
+    specs = synthetic.serving_mix(args.serve_jobs, seed=5)
+    log(f"serving mix: {len(specs)} jobs generated in {time.time()-t0:.1f}s")
+    budget = 200 << 30
+    ws = 40 << 30
+    srv = serving.Server(budget, n_streams=args.serve_streams, ws_bytes_per_stream=ws,
+                         device=dev.index or 0)
+    jobs32, jobs64 = [], []
+    for sp in specs:
+        post = [tuple(int(x) for x in r) for r in sp["topo"].postorder]
+        T, N, S, C = sp["topo"].T, sp["codes"].shape[1], sp["S"], len(sp["rates"])
+        common = (post, sp["codes"], sp["table"], sp["weights"], sp["q_norm"], sp["freqs"],
+                  sp["rates"], sp["props"])
+        j32 = serving.ResidentJob(sp["name"], *common, dtype=torch.float32,
+                                  slice_patterns=srv.slice_for(T, N, C, S, torch.float32),
+                                  device=dev.index or 0, kind=sp["kind"])
+        srv.register(sp["name"], j32)
+        jobs32.append(j32)
+        jobs64.append(serving.ResidentJob(sp["name"], *common, dtype=torch.float64,
+                                          slice_patterns=srv.slice_for(T, N, C, S, torch.float64),
+                                          device=dev.index or 0, kind=sp["kind"]))
+    log(f"registered {len(jobs32)} resident jobs ({srv.resident / 2**30:.2f} GiB FP32 tips) in "
+        f"{time.time()-t0:.1f}s")
+
+    def requests(jobs, stream):
+        return [serving.Request(i, jobs[j], synthetic.draw_bls(specs[j], K, sd).pin_memory(),
+                                arrival=t) for i, (j, K, sd, t) in enumerate(stream)]
+
+    n = args.serve_requests
+    # capacity: all requests queued at t=0
+    burst = synthetic.request_stream(n, len(specs), rate=1e9, seed=7)
+    warm = requests(jobs32, burst[: min(20, n)])
+    srv.run(warm, realtime=False)
+    prog = lambda d, t: log(f"  ... {d}/{t} requests done")
+    cap = srv.run(requests(jobs32, burst), realtime=False, progress=prog)
+    mean_upd = cap["updates"] / cap["requests"]
+    rate = 0.8 * cap["updates_per_s"] / mean_upd
+    log(f"serving capacity {cap['updates_per_s']:.3e} updates/s; offered rate {rate:.1f} req/s")
+    stream = synthetic.request_stream(n, len(specs), rate=rate, seed=8)
+    served = requests(jobs32, stream)
+    st = srv.run(served, progress=prog)
+    log(f"served: {st}")
+    plain = requests(jobs64, stream)
+    serving.run_sequential(plain[: min(n, 5)], device=dev.index or 0)  # warm-up
+    plain = requests(jobs64, stream)
+    pl = serving.run_sequential(plain, device=dev.index or 0, time_limit=args.serve_plain_limit,
+                                progress=prog)
+    log(f"plain fp64 sequential: {pl}")
+    errs, gerrs = [], []
+    for a, b in zip(served, plain):
+        if b.loglik is None or math.isnan(b.t_done):
+            continue
+        errs.append(float(((a.loglik - b.loglik).abs() / b.loglik.abs()).max()))
+        gerrs.append(float((a.dbls - b.dbls).norm() / b.dbls.norm()))
+    kinds = [r.job.kind for r in served]
+    return {
+        "workload": f"{len(specs)} resident jobs (T~U{{16..1024}}, N log-uniform 100-100000, "
+                    f"DNA/AA/codon {kinds.count('dna')}/{kinds.count('aa')}/{kinds.count('codon')} "
+                    f"requests), K in {{1,8,32}}, Poisson arrivals at {rate:.1f}/s (80% of capacity), "
+                    f"{args.serve_streams} streams",
+        "requests": st["requests"],
+        "served_updates_per_s": st["updates_per_s"],
+        "served_p50_ms": st["p50_ms"], "served_p99_ms": st["p99_ms"],
+        "capacity_updates_per_s": cap["updates_per_s"],
+        "plain_fp64_sequential_updates_per_s": pl["updates_per_s"],
+        "plain_fp64_requests_timed": pl["requests"],
+        "plain_fp64_p50_ms": pl["p50_ms"], "plain_fp64_p99_ms": pl["p99_ms"],
+        "served_vs_plain": st["updates_per_s"] / pl["updates_per_s"],
+        "capacity_vs_plain": cap["updates_per_s"] / pl["updates_per_s"],
+        "max_logl_rel_err_vs_fp64": max(errs), "max_grad_rel_err_vs_fp64": max(gerrs),
+    }
 
 
 def _nchunks(N, C, K):

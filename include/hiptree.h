@@ -187,6 +187,11 @@ size_t hx_job_workspace_bytes(hx_job_t job, int K, int want_grad);
 int hx_job_eval(hx_job_t job, int K, const double* bls, double* out_loglik, double* out_dbls,
                 void* workspace, size_t workspace_bytes, hx_stream_t stream);
 
+/* Checks on the device the operand/accumulator lane layout the MFMA
+ * pruning kernels assume for v_mfma_f32_16x16x4f32; *mismatches = number of
+ * wrong output elements (0 expected). No reference counterpart. */
+int hx_selftest_mfma_layout(int* mismatches, hx_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

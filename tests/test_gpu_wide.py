@@ -129,3 +129,43 @@ def test_engine_mg94(gpu, dtype, rl, rg):
     assert float((b2.grad.cpu() - bls.grad).norm() / bls.grad.norm()) < rg
     assert abs(float(k2.grad) - float(kappa.grad)) < rg * abs(float(kappa.grad)) + 1e-9
     assert abs(float(o2.grad) - float(omega.grad)) < rg * abs(float(omega.grad)) + 1e-9
+
+
+def test_mfma_lane_layout(gpu):
+    """The 16x16x4 FP32 MFMA operand/result lane layout the dense kernels assume."""
+    print("case: mfma layout selftest", flush=True)
+    import ctypes
+
+    from torchtree_amd import _lib
+
+    lib = _lib.load()
+    m = ctypes.c_int(-1)
+    _lib.check(lib.hx_selftest_mfma_layout(ctypes.byref(m), _lib.stream_handle(None)))
+    torch.cuda.synchronize()
+    assert m.value == 0
+
+
+@pytest.mark.parametrize("S,C,N", [(20, 3, 1000), (61, 2, 333), (17, 1, 515), (33, 4, 70)])
+def test_dropin_wide_many_tiles(gpu, S, C, N):
+    """FP32 dense kernels over many pattern tiles and chunks, ragged last tile."""
+    print(f"case: wide many tiles S={S} C={C} N={N}", flush=True)
+    from torchtree_amd import likelihood as hl
+
+    T, K = 12, 3
+    post, tips, codes, w, pi, ex, mats, rates, props, bl = _random_case(S, T, N, K, C, S + N + C)
+    m = mats.detach().clone().requires_grad_(True)
+    f = pi.reshape(1, -1).clone().requires_grad_(True)
+    p = props.reshape(-1, 1, 1).clone().requires_grad_(True)
+    ref = opruning.treelik_discrete_rescaled(tips + [None] * (T - 1), w, post, m, f, p)
+    ref.sum().backward()
+    m2 = mats.detach().to(gpu, torch.float32).requires_grad_(True)
+    f2 = pi.reshape(1, -1).to(gpu, torch.float32).requires_grad_(True)
+    p2 = props.reshape(-1, 1, 1).to(gpu, torch.float32).requires_grad_(True)
+    got = hl.calculate_treelikelihood_discrete(tips + [None] * (T - 1), w, post, m2, f2, p2)
+    rel = float(((got.detach().double().cpu() - ref.detach()).abs() / ref.detach().abs()).max())
+    assert rel < 2e-6, rel
+    got.sum().backward()
+    for a, b in ((m2, m), (f2, f), (p2, p)):
+        ga, gb = a.grad.double().cpu(), b.grad
+        err = float((ga - gb).norm() / gb.norm())
+        assert err < 2e-5, err

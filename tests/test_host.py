@@ -148,3 +148,40 @@ def test_json_objects_parse(tiny_newick, tiny_fasta):
     assert torch.allclose(r, torch.tensor([0.1457844, 0.5131316, 1.0708310, 2.2702530], dtype=torch.float64), rtol=1e-6)
     with pytest.raises(tm.JSONParseError):
         tm.process_object("missing", dic)
+
+
+def test_integration_install_roundtrip():
+    """install() swaps exactly the six pruning functions and uninstall() restores them."""
+    import types
+
+    from torchtree_amd import integration, likelihood
+
+    mod = types.ModuleType("tree_likelihood_standin")
+    originals = {}
+    for n in integration.NAMES:
+        f = (lambda name: (lambda *a, **k: name))(n)
+        setattr(mod, n, f)
+        originals[n] = f
+    mod.other = 1
+    saved = integration.install(mod)
+    assert saved == originals
+    for n in integration.NAMES:
+        assert getattr(mod, n) is getattr(likelihood, n)
+    assert mod.other == 1
+    integration.uninstall(mod, saved)
+    for n in integration.NAMES:
+        assert getattr(mod, n) is originals[n]
+
+
+def test_integration_install_rejects_other_modules():
+    import types
+
+    import pytest
+
+    from torchtree_amd import integration
+
+    mod = types.ModuleType("unrelated")
+    mod.calculate_treelikelihood = 1
+    with pytest.raises(AttributeError):
+        integration.install(mod)
+    assert mod.calculate_treelikelihood == 1

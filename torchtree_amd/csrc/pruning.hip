@@ -28,6 +28,8 @@
 
 #include <math.h>
 
+#include <type_traits>
+
 namespace hx {
 
 template <typename R, int S>
@@ -502,6 +504,16 @@ static int dispatch_cp(bool post, const PassArgs<R>& a, const int32_t* nodes, in
   return HX_OK;
 }
 
+template <typename R>
+static int launch_dense(bool post, const PassArgs<R>& a, const int32_t* nodes, int width, int tips,
+                        bool root, int S, hipStream_t st) {
+  if constexpr (std::is_same<R, float>::value) {
+    if (use_mfma(a.C, S, HX_F32)) return launch_mfma(post, a, nodes, width, tips, root, S, st);
+  }
+  return post ? launch_post_wide<R>(a, nodes, width, tips, root, S, st)
+              : launch_pre_wide<R>(a, nodes, width, tips, root, S, st);
+}
+
 struct CallArgs {
   hx_plan_t plan;
   int S, N, C, K, dtype, tip_kind;
@@ -586,7 +598,7 @@ static int run_post(const CallArgs& c, double* out_l, double* out_f, double* out
     const int off = plan->post_off[l], width = plan->post_off[l + 1] - off;
     int rc = c.S == 4
                  ? dispatch_cp<R, 4>(true, a, plan->d_post + off, width, c.tip_kind, l == H - 1, st)
-                 : launch_post_wide<R>(a, plan->d_post + off, width, c.tip_kind, l == H - 1, c.S, st);
+                 : launch_dense<R>(true, a, plan->d_post + off, width, c.tip_kind, l == H - 1, c.S, st);
     if (rc != HX_OK) return rc;
   }
   const int Q = 1 + c.S + c.C;
@@ -606,7 +618,7 @@ static int run_pre(const CallArgs& c, double* out, hipStream_t st) {
     const int off = plan->pre_off[d], width = plan->pre_off[d + 1] - off;
     int rc = c.S == 4
                  ? dispatch_cp<R, 4>(false, a, plan->d_pre + off, width, c.tip_kind, d == 0, st)
-                 : launch_pre_wide<R>(a, plan->d_pre + off, width, c.tip_kind, d == 0, c.S, st);
+                 : launch_dense<R>(false, a, plan->d_pre + off, width, c.tip_kind, d == 0, c.S, st);
     if (rc != HX_OK) return rc;
   }
   if (c.grad_mode == 2) {

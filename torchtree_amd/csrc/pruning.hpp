@@ -39,4 +39,9 @@ template <typename R>
 int launch_pre_wide(const PassArgs<R>& a, const int32_t* nodes, int width, int tips, bool root,
                     int S, hipStream_t st);
 
+// FP32 dense-state kernels on MFMA (pruning_mfma.hip); used when use_mfma().
+bool use_mfma(int C, int S, int dtype);
+int launch_mfma(bool post, const PassArgs<float>& a, const int32_t* nodes, int width, int tips,
+                bool root, int S, hipStream_t st);
+
 }  // namespace hx

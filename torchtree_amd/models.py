@@ -14,9 +14,10 @@ inputs of the hot path, following torchtree 1.0.2:
 * ``gamma_rates`` -> new (torchtree has no gamma model), same quantile-midpoint
   contract, Gamma(shape, rate=shape)
 
-``TransitionFn`` is the autograd function of the native path: FP64 eigen on
-the host, ``hx_pmatrix`` for P on the device, and ``hx_pmatrix_grad`` for the
-chain rule back to branch lengths, category rates and Q.
+``Eigen``, ``pmatrix`` and ``pmatrix_grad`` are the transition-matrix stage of
+the native path (driven by ``engine._TreeLikFn``): FP64 eigen on the host,
+``hx_pmatrix`` for P on the device, and ``hx_pmatrix_grad`` for the chain rule
+back to branch lengths, category rates and Q.
 """
 from __future__ import annotations
 

@@ -260,15 +260,23 @@ class Server:
         job = self.jobs.pop(name)
         self.resident -= job.resident_bytes
 
-    def run(self, requests, realtime=True):
-        """Serve ``requests`` (sorted by arrival). Returns a stats dict."""
+    def run(self, requests, realtime=True, progress=None, every=5.0):
+        """Serve ``requests`` (sorted by arrival). Returns a stats dict.
+
+        ``progress(done, total)`` is called at most every ``every`` seconds.
+        """
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         queue = collections.deque()
         slots = [None] * self.executor.n
         t0 = self.clock()
         done = []
+        last = t0
+        total = len(pending)
         while pending or queue or any(s is not None for s in slots):
             progressed = False
+            if progress is not None and self.clock() - last > every:
+                last = self.clock()
+                progress(len(done), total)
             now = self.clock() - t0
             while pending and (pending[0].arrival <= now or not realtime):
                 queue.append(pending.popleft())
@@ -308,14 +316,27 @@ def summarize(done):
     }
 
 
-def run_sequential(requests, device=0, ws_bytes=None):
-    """Plain per-call evaluation: one request at a time, synchronised, back to back."""
+def run_sequential(requests, device=0, ws_bytes=None, time_limit=None, progress=None,
+                   every=5.0):
+    """Plain per-call evaluation: one request at a time, synchronised, back to back.
+
+    With ``time_limit`` (seconds) the run stops after the request that crosses
+    it; the stats then cover the completed requests only (``partial``).
+    """
     dev = torch.device("cuda", device)
     need = max(r.job.workspace_bytes(r.K, r.grad) for r in requests)
     ws = torch.empty(int(need if ws_bytes is None else ws_bytes), dtype=torch.uint8, device=dev)
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
+    last = t0
+    done = []
     for r in requests:
+        if time_limit is not None and time.perf_counter() - t0 > time_limit:
+            break
+        if progress is not None and time.perf_counter() - last > every:
+            last = time.perf_counter()
+            progress(len(done), len(requests))
+        done.append(r)
         a = time.perf_counter()
         bls = r.bls.to(dev)
         r.loglik = torch.empty(r.K, dtype=F64, device=dev)
@@ -325,4 +346,6 @@ def run_sequential(requests, device=0, ws_bytes=None):
         r.t_start = a - t0
         r.arrival = a - t0
         r.t_done = time.perf_counter() - t0
-    return summarize(requests)
+    out = summarize(done)
+    out["partial"] = len(done) < len(requests)
+    return out
