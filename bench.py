@@ -64,17 +64,50 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(T, N, C, K, S, rb, nchunks):
-    """Bytes each pass must move at minimum with this algorithm (one launch set)."""
+def even_depth_internal(post, T):
+    """Internal non-root nodes at even depth >= 2: the pre-order group roots
+    whose U is written and read back by the two-level pre-order kernel."""
+    kids = {v: (a, b) for v, a, b in post}
+    root = post[-1][0]
+    depth = {root: 0}
+    for v, a, b in reversed(post):
+        depth[a] = depth[v] + 1
+        depth[b] = depth[v] + 1
+    return sum(1 for v in kids if v != root and depth[v] % 2 == 0)
+
+
+def walk_need(post, T):
+    """Stack entries of the post-order walk (larger-need child first)."""
+    need = {}
+    for v, a, b in post:
+        na, nb = need.get(a, 0), need.get(b, 0)
+        need[v] = max(na, nb, 1 + min(na, nb)) if (na and nb) else max(1, na, nb)
+    return need[post[-1][0]]
+
+
+def algorithmic_bytes(post, T, N, C, K, S, rb, nchunks, pre_levels):
+    """Bytes each pass must move at minimum with this algorithm (one launch set).
+
+    P = K N C S rb is one node's partials. Post-order walk: every internal
+    non-root partial written once, nothing read back, tip codes once. Pre-order
+    over `pre_levels` levels per launch: every internal non-root L read once;
+    with two levels only the group roots (even depth >= 2) have U written and
+    read back, with one level every internal non-root U is.
+    """
     part = K * N * C * S * rb
-    scale = K * N * 4
+    I = T - 1
     mats = K * (2 * T - 2) * C * S * S * rb
     tips = T * N * 1
     weights = N * 8
-    post = 2 * (T - 2) * part + 2 * (T - 2) * scale + tips + mats + weights
     gpart = (2 * T - 2) * K * nchunks * C * S * S * rb
-    pre = 3 * (T - 2) * part + tips + mats + weights + gpart
-    return post, pre
+    if S != 4:  # dense-state level kernels: written once, read once by the parent
+        post_b = 2 * (I - 1) * part + 2 * (I - 1) * K * N * 4 + tips + mats + weights
+        pre_b = 3 * (I - 1) * part + tips + mats + weights + gpart
+        return post_b, pre_b
+    post_b = (I - 1) * part + tips + mats + weights
+    nu = even_depth_internal(post, T) if pre_levels == 2 else I - 1
+    pre_b = (I - 1) * part + 2 * nu * part + tips + mats + weights + gpart
+    return post_b, pre_b
 
 
 def main():
@@ -154,7 +187,8 @@ def main():
     ll32 = ll32.detach().cpu()
     g32 = (b32.grad.detach().cpu(), e32.grad.detach().cpu(), f32.grad.detach().cpu())
     nchunks = _nchunks(wl.N, wl.C, wl.K)
-    post_b, pre_b = algorithmic_bytes(wl.T, wl.N, wl.C, wl.K, S, 4, nchunks)
+    pre_levels = 2 if (S == 4 and wl.C <= 4) else 1
+    post_b, pre_b = algorithmic_bytes(post, wl.T, wl.N, wl.C, wl.K, S, 4, nchunks, pre_levels)
     post_ms = phases["postorder"]["ms"]
     pre_ms = phases["preorder"]["ms"]
     post_gbs = post_b / (post_ms * 1e-3) / 1e9
@@ -188,14 +222,16 @@ def main():
         "updates_per_step": upd,
         "phases_ms": {k: v["ms"] for k, v in phases.items()},
         "roofline": {
-            "bound": "hbm", "kernel": f"k_pre (all {launches} depth-level launches)" if dom == "preorder"
-            else f"k_post (all {launches} height-level launches)",
+            "bound": "hbm",
+            "kernel": (f"k_pre_grp ({launches - 1} launches, {pre_levels} tree levels each) + k_reduce_g"
+                       if dom == "preorder" else "k_post_walk (one launch, whole tree) + k_reduce_root"),
             "achieved": dom_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dom_gbs / HBM_PEAK_GBS,
             "traffic": None, "algorithmic_bytes": dom_b, "pass_ms": dom_ms,
             "avg_launch_us": dom_ms * 1e3 / max(1, launches - 1),
         },
         "other_pass": {"postorder_gbs": post_gbs, "preorder_gbs": pre_gbs,
-                       "postorder_bytes": post_b, "preorder_bytes": pre_b},
+                       "postorder_bytes": post_b, "preorder_bytes": pre_b,
+                       "walk_stack_entries": walk_need(post, wl.T)},
     }
 
     # ---- FP64 plain path: accuracy reference at full size, and its throughput
@@ -327,10 +363,7 @@ def serving_bench(args, dev):
 
 def _nchunks(N, C, K):
     """Chunk count of the S=4 kernels (mirrors make_tiling in csrc/common.hpp)."""
-    cp = 1
-    while cp < C:
-        cp *= 2
-    ppb = 256 // cp
+    ppb = 64
     ntiles = (N + ppb - 1) // ppb
     nch = max(1, min(ntiles, 128))
     tpc = (ntiles + nch - 1) // nch

@@ -1,5 +1,6 @@
 // C ABI: status strings, topology plans and workspace sizing.
 #include "common.hpp"
+#include "walk.hpp"
 
 #include <algorithm>
 #include <string>
@@ -94,13 +95,22 @@ int hx_plan_create(int taxa, const int32_t* postorder, int device, hx_plan_t* ou
     post_lv[height[v] - 1].push_back(v);
     pre_lv[depth[v]].push_back(v);
   }
+  std::vector<int32_t> walk;
+  int walk_need = 0;
+  if (!hx::build_walk(T, kids, postorder, walk, walk_need)) {
+    hx::set_error("walk program construction failed");
+    return HX_ERR_BAD_TOPOLOGY;
+  }
   auto* plan = new hx_plan_s();
   plan->device = device;
+  plan->walk_need = walk_need;
   plan->T = T;
   plan->I = I;
   plan->root = root;
   plan->kids = kids;
-  std::vector<int32_t> host(2 * nn + 2 * I);
+  const size_t walk_off = hx::align_up(2 * (size_t)nn + 2 * (size_t)I, 4);
+  std::vector<int32_t> host(walk_off + walk.size(), 0);
+  std::copy(walk.begin(), walk.end(), host.begin() + walk_off);
   std::copy(kids.begin(), kids.end(), host.begin());
   int pos = 2 * nn;
   plan->post_off.push_back(0);
@@ -129,6 +139,7 @@ int hx_plan_create(int taxa, const int32_t* postorder, int device, hx_plan_t* ou
   plan->d_kids = plan->d_buf;
   plan->d_post = plan->d_buf + 2 * nn;
   plan->d_pre = plan->d_buf + 2 * nn + I;
+  plan->d_walk = plan->d_buf + walk_off;
   *out = plan;
   return HX_OK;
 }

@@ -57,8 +57,9 @@ struct Tiling {
 
 int wide_nt(int C, int S, int rb);
 
-// S == 4 uses lane-per-(pattern, category) tiles of 256/cp patterns; other S
-// use tiles of wide_nt() patterns handled by (child, category, state) lanes.
+// S == 4 uses tiles of 64 patterns (a wave per category, a lane per pattern);
+// other S use tiles of wide_nt() patterns handled by (child, category, state)
+// lanes.
 // The chunking depends on N (and the tile shape) only, never on K: every
 // sample's sums run in the same order whatever batch it is evaluated in, so a
 // sample's result is bitwise identical alone and in a batch.
@@ -70,7 +71,7 @@ inline Tiling make_tiling(int N, int C, int K, int S, int rb) {
   t.cp = next_pow2(C < 1 ? 1 : C);
   int min_tiles = 1;
   if (S == 4) {
-    t.ppb = kBlock / t.cp;
+    t.ppb = kWave;  // one wave per category, one lane per pattern
   } else {
     t.ppb = wide_nt(C, S, rb);
     min_tiles = 4;
@@ -92,11 +93,18 @@ struct hx_plan_s {
   int T = 0;
   int I = 0;
   int root = 0;
-  int32_t* d_buf = nullptr;   // device: kids[2(2T-1)] | post_nodes[I] | pre_nodes[I]
+  int32_t* d_buf = nullptr;   // device: kids[2(2T-1)] | post_nodes[I] | pre_nodes[I] | pad |
+                              //         walk[I][4] (16-byte aligned)
   const int32_t* d_kids = nullptr;
   const int32_t* d_post = nullptr;
   const int32_t* d_pre = nullptr;
   std::vector<int> post_off;  // host: level offsets into d_post (size levels+1)
   std::vector<int> pre_off;   // host: depth offsets into d_pre
   std::vector<int32_t> kids;  // host copy of kids
+  // Post-order walk program, int4 per internal node (node, left, right,
+  // slots): slots = (left slot + 1) | (right slot + 1) << 8 | out slot << 16,
+  // slot 0 = tip. The larger-need child is evaluated first, so the walk
+  // needs `walk_need` stack entries (at most about log2(T) + 1).
+  const int32_t* d_walk = nullptr;
+  int walk_need = 0;
 };

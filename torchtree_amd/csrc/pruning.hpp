@@ -15,14 +15,14 @@ struct PassArgs {
   const R* table;         // [ncodes][S]
   int ncodes;
   const R* dense;         // [T][N][S]
-  R* partials;            // [I][K][N][C][S]
+  R* partials;            // [I][K][N][C][S]; S = 4: [I][K][C][N][4]
   int32_t* scales;        // [I][K][N]
   const R* freqs;
   int freqs_ks;           // stride between samples (0 or S)
   const R* props;
   int props_ks;           // stride between samples (0 or C)
   const double* weights;  // [N]
-  double* rootsums;       // [K][nchunks][1+S+C]
+  double* rootsums;       // [K][nchunks][1+S+C]; S = 4: [K][ntiles][1+S+C]
   R* gpart;               // [B][K][nchunks][C][S][S]     (d loglik / d mats mode)
   const R* mats_dt;       // [K][B][C][S][S] dP/dtau; non-null selects d loglik / d tau mode
   double* gtau;           // [B][K][nchunks][C]            (d loglik / d tau mode)
@@ -38,6 +38,19 @@ int launch_post_wide(const PassArgs<R>& a, const int32_t* nodes, int width, int 
 template <typename R>
 int launch_pre_wide(const PassArgs<R>& a, const int32_t* nodes, int width, int tips, bool root,
                     int S, hipStream_t st);
+
+// Four-state kernels (pruning_dna.hip): fused post-order walk and the
+// pre-order adjoint over `dna_pre_levels` tree levels per launch. Partials of
+// S = 4 use the category-major layout [slot][K][C][N][4].
+constexpr size_t kDnaLdsBudget = 64 * 1024;
+template <typename R>
+int launch_post_dna(const PassArgs<R>& a, const int32_t* prog, int nprog, int need, int tips,
+                    hipStream_t st);
+template <typename R>
+int launch_pre_dna(const PassArgs<R>& a, const int32_t* nodes, int width, int tips, bool root,
+                   hipStream_t st);
+template <typename R>
+int dna_pre_levels(int C);
 
 // FP32 dense-state kernels on MFMA (pruning_mfma.hip); used when use_mfma().
 bool use_mfma(int C, int S, int dtype);
