@@ -156,3 +156,43 @@ def test_engine_grad_model(gpu, dtype, rtol_l, rtol_g):
                        (freqs2.grad, freqs.grad, "freqs")):
         err = float((a - b).norm() / b.norm())
         assert err < rtol_g, (name, err)
+
+
+@pytest.mark.parametrize("T,N,C,dense", [(70, 700, 8, False), (40, 260, 16, False), (30, 333, 4, True)])
+@pytest.mark.parametrize("dtype,rtol_l,rtol_g", [(F64, 1e-12, 1e-10), (torch.float32, 1e-6, 1e-5)])
+def test_dropin_dna_shapes(gpu, T, N, C, dense, dtype, rtol_l, rtol_g):
+    """Four-state kernels beyond the default shape: 8 and 16 categories (one-level
+    pre-order; at 16 the walk stack no longer fits in LDS and spills to HBM) and
+    dense tip partials (more than 256 distinct tip columns)."""
+    _flush(f"case: drop-in DNA T={T} N={N} C={C} dense={dense} {dtype}")
+    from torchtree_amd import likelihood as hl
+    from torchtree_amd import synthetic
+
+    wl = synthetic.make_workload(T, N, 2, categories=C, model="gtr", seed=T + C, bl_mean=0.1)
+    post = [tuple(int(x) for x in r) for r in wl.topo.postorder]
+    tips = wl.tip_partials()
+    if dense:
+        rng = np.random.default_rng(5)
+        tips = [t * torch.tensor(rng.uniform(0.2, 1.0, t.shape), dtype=t.dtype) + 0.01 for t in tips]
+        assert hl.TipData.from_partials(tips, "cpu").kind == hl.HX_TIPS_DENSE
+    q = omodels.gtr_q(wl.exch, wl.freqs)
+    rates = torch.linspace(0.2, 2.5, C, dtype=F64)
+    props = torch.full((C,), 1.0 / C, dtype=F64)
+    tau = wl.bls.unsqueeze(-1) * rates.reshape(1, 1, -1)
+    mats = omodels.p_t_eigh(q, wl.freqs, tau).detach().requires_grad_(True)
+    freqs = wl.freqs.reshape(1, -1).clone().requires_grad_(True)
+    pr = props.reshape(-1, 1, 1).clone().requires_grad_(True)
+    ref = opruning.treelik_discrete_rescaled(tips + [None] * (wl.T - 1), wl.weights, post, mats,
+                                             freqs, pr)
+    ref.sum().backward()
+    m2 = mats.detach().to(gpu, dtype).requires_grad_(True)
+    f2 = freqs.detach().to(gpu, dtype).requires_grad_(True)
+    p2 = pr.detach().to(gpu, dtype).requires_grad_(True)
+    got = hl.calculate_treelikelihood_discrete(tips + [None] * (wl.T - 1), wl.weights, post, m2, f2, p2)
+    rel = (got.double().cpu() - ref.detach()).abs() / ref.detach().abs()
+    assert float(rel.max()) < rtol_l
+    got.sum().backward()
+    for a, b in ((m2.grad, mats.grad), (f2.grad, freqs.grad), (p2.grad, pr.grad)):
+        a = a.double().cpu()
+        err = float((a - b).norm() / b.norm())
+        assert err < rtol_g, err

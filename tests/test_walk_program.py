@@ -32,28 +32,44 @@ def walk_bin(tmp_path_factory):
 def _run(walk_bin, T, post):
     inp = f"{T}\n" + "\n".join(f"{v} {a} {b}" for v, a, b in post) + "\n"
     r = subprocess.run([str(walk_bin)], input=inp, capture_output=True, text=True, check=True)
-    lines = r.stdout.split("\n")
-    need = int(lines[0])
-    prog = [tuple(int(x) for x in ln.split()) for ln in lines[1:] if ln.strip()]
-    return need, prog
+    lines = [ln for ln in r.stdout.split("\n") if ln.strip()]
+    need, nseg = (int(x) for x in lines[0].split())
+    prog = [tuple(int(x) for x in ln.split()) for ln in lines[1:T]]
+    segtab = [[int(x) for x in ln.split()] for ln in lines[T:]]
+    assert len(segtab) == nseg
+    return need, (prog, segtab)
 
 
-def _check(T, post, need, prog):
+def _check(T, post, need, walk):
+    prog, segtab = walk
     kids = {v: (a, b) for v, a, b in post}
     assert len(prog) == T - 1
     assert prog[-1][0] == post[-1][0] == 2 * T - 2
     leaves = {t: frozenset([t]) for t in range(T)}  # value of a node = its leaf set
     stack = [None] * need
     done = set()
-    for v, a, b, slots in prog:
+    seg = -1
+    used = []
+    prev = None
+    for v, a, b, slots, tpos, sopen, kinds in prog:
         assert kids[v] == (a, b) and v not in done
         sa, sb, so = (slots & 0xFF) - 1, ((slots >> 8) & 0xFF) - 1, slots >> 16
+        pa, pb = (tpos & 0xFF) - 1, ((tpos >> 8) & 0xFF) - 1
+        if sopen >= 0:
+            assert sopen == seg + 1  # segments open in order
+            seg = sopen
+        assert seg >= 0
         vals = []
-        for ch, sl in ((a, sa), (b, sb)):
+        for ch, sl, pos, kind in ((a, sa, pa, kinds & 0xFF), (b, sb, pb, kinds >> 8)):
+            # kind: 0 tip, 1 result of the previous entry, 2 stacked earlier result
+            assert kind == (0 if ch < T else (1 if ch == prev else 2))
             if ch < T:
-                assert sl == -1
+                assert sl == -1 and 0 <= pos < 16
+                assert segtab[seg][pos] == ch  # the tip's codes are staged in that row
+                used.append(ch)
                 vals.append(leaves[ch])
             else:
+                assert pos == -1
                 assert 0 <= sl < need and ch in done
                 assert stack[sl] == leaves[ch]  # the slot still holds that child's value
                 vals.append(stack[sl])
@@ -65,7 +81,11 @@ def _check(T, post, need, prog):
         stack[so] = val
         leaves[v] = val
         done.add(v)
+        prev = v
     assert leaves[2 * T - 2] == frozenset(range(T))
+    assert sorted(used) == list(range(T))  # every tip staged and consumed once
+    assert seg == len(segtab) - 1
+    assert sorted(t for row in segtab for t in row if t >= 0) == list(range(T))
     assert need <= int(math.floor(math.log2(T))) + 1
 
 

@@ -95,9 +95,9 @@ int hx_plan_create(int taxa, const int32_t* postorder, int device, hx_plan_t* ou
     post_lv[height[v] - 1].push_back(v);
     pre_lv[depth[v]].push_back(v);
   }
-  std::vector<int32_t> walk;
+  std::vector<int32_t> walk, segtab;
   int walk_need = 0;
-  if (!hx::build_walk(T, kids, postorder, walk, walk_need)) {
+  if (!hx::build_walk(T, kids, postorder, walk, segtab, walk_need)) {
     hx::set_error("walk program construction failed");
     return HX_ERR_BAD_TOPOLOGY;
   }
@@ -109,8 +109,10 @@ int hx_plan_create(int taxa, const int32_t* postorder, int device, hx_plan_t* ou
   plan->root = root;
   plan->kids = kids;
   const size_t walk_off = hx::align_up(2 * (size_t)nn + 2 * (size_t)I, 4);
-  std::vector<int32_t> host(walk_off + walk.size(), 0);
+  const size_t seg_off = walk_off + walk.size();
+  std::vector<int32_t> host(seg_off + segtab.size(), 0);
   std::copy(walk.begin(), walk.end(), host.begin() + walk_off);
+  std::copy(segtab.begin(), segtab.end(), host.begin() + seg_off);
   std::copy(kids.begin(), kids.end(), host.begin());
   int pos = 2 * nn;
   plan->post_off.push_back(0);
@@ -140,6 +142,8 @@ int hx_plan_create(int taxa, const int32_t* postorder, int device, hx_plan_t* ou
   plan->d_post = plan->d_buf + 2 * nn;
   plan->d_pre = plan->d_buf + 2 * nn + I;
   plan->d_walk = plan->d_buf + walk_off;
+  plan->d_segtab = plan->d_buf + seg_off;
+  plan->nseg = (int)(segtab.size() / hx::kWalkSegTips);
   *out = plan;
   return HX_OK;
 }

@@ -94,17 +94,17 @@ struct hx_plan_s {
   int I = 0;
   int root = 0;
   int32_t* d_buf = nullptr;   // device: kids[2(2T-1)] | post_nodes[I] | pre_nodes[I] | pad |
-                              //         walk[I][4] (16-byte aligned)
+                              //         walk[I][8] (16-byte aligned) | segtab[nseg][16]
   const int32_t* d_kids = nullptr;
   const int32_t* d_post = nullptr;
   const int32_t* d_pre = nullptr;
   std::vector<int> post_off;  // host: level offsets into d_post (size levels+1)
   std::vector<int> pre_off;   // host: depth offsets into d_pre
   std::vector<int32_t> kids;  // host copy of kids
-  // Post-order walk program, int4 per internal node (node, left, right,
-  // slots): slots = (left slot + 1) | (right slot + 1) << 8 | out slot << 16,
-  // slot 0 = tip. The larger-need child is evaluated first, so the walk
-  // needs `walk_need` stack entries (at most about log2(T) + 1).
+  // Post-order walk program (walk.hpp): kWalkInts ints per internal node and
+  // the tip-segment table; the walk needs `walk_need` stack entries.
   const int32_t* d_walk = nullptr;
+  const int32_t* d_segtab = nullptr;
+  int nseg = 0;
   int walk_need = 0;
 };

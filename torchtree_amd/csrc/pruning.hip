@@ -11,8 +11,8 @@
 // Layout in HBM (R = float or double):
 //   partials [slot][K][N][C][S]   slot = node - T (S != 4), pattern-major so a
 //                                 pattern's C*S values are contiguous;
-//            [slot][K][C][N][4]   for S = 4, category-major so one wave (one
-//                                 category, 64 patterns) reads 1 KiB runs
+//            [slot][K][C][np][4]  for S = 4, category-major so a wave reads
+//                                 1 KiB runs of one category; np = 64 ntiles
 //   scales   [slot][K][N] int32   cumulative power-of-two exponent of the
 //                                 subtree below the node
 //   rootsums [K][part][1+S+C]     FP64 partial sums at the root per chunk
@@ -105,9 +105,11 @@ static WsLayout ws_layout(const hx_plan_s* plan, int S, int N, int C, int K, int
   WsLayout w;
   size_t off = 0;
   w.partials = off;
-  off = align_up(off + I * K * (size_t)N * C * S * rb, 256);
-  w.scales = off;
-  off = align_up(off + I * K * (size_t)N * 4, 256);
+  const size_t np = S == 4 ? (size_t)t.ntiles * 64 : (size_t)N;
+  off = align_up(off + I * K * np * C * S * rb, 256);
+  w.scales = off;  // S = 4: 64-byte-aligned copy of the tip codes [T][ntiles * 64]
+  if (S == 4) off = align_up(off + (size_t)plan->T * t.ntiles * 64, 256);
+  else off = align_up(off + I * K * (size_t)N * 4, 256);
   w.rootsums = off;
   off = align_up(off + (size_t)K * t.ntiles * (1 + S + C) * 8, 256);
   w.gpart = off;
@@ -183,6 +185,7 @@ static PassArgs<R> make_args(const CallArgs& c) {
   a.ntiles = t.ntiles;
   a.tiles_per_chunk = t.tiles_per_chunk;
   a.nchunks = t.nchunks;
+  a.np = c.S == 4 ? t.ntiles * 64 : c.N;
   a.mats = static_cast<const R*>(c.mats);
   a.codes = c.tip_kind == HX_TIPS_CODES ? static_cast<const uint8_t*>(c.tips) : nullptr;
   a.table = static_cast<const R*>(c.table);
@@ -190,6 +193,7 @@ static PassArgs<R> make_args(const CallArgs& c) {
   a.dense = c.tip_kind == HX_TIPS_DENSE ? static_cast<const R*>(c.tips) : nullptr;
   a.partials = reinterpret_cast<R*>(base + w.partials);
   a.scales = reinterpret_cast<int32_t*>(base + w.scales);
+  a.codes_pad = reinterpret_cast<uint8_t*>(base + w.scales);
   a.freqs = static_cast<const R*>(c.freqs);
   a.freqs_ks = c.freqs_batched ? c.S : 0;
   a.props = static_cast<const R*>(c.props);
@@ -208,7 +212,8 @@ static int run_post(const CallArgs& c, double* out_l, double* out_f, double* out
   const hx_plan_s* plan = c.plan;
   int nparts = a.nchunks;
   if (c.S == 4) {
-    const int rc = launch_post_dna<R>(a, plan->d_walk, plan->I, plan->walk_need, c.tip_kind, st);
+    const int rc = launch_post_dna<R>(a, plan->d_walk, plan->d_segtab, plan->nseg, plan->I,
+                                      plan->walk_need, c.tip_kind, st);
     if (rc != HX_OK) return rc;
     nparts = a.ntiles;
   } else {

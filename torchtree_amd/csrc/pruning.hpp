@@ -10,13 +10,15 @@ struct PassArgs {
   const int32_t* kids;  // [2T-1][2]
   int T, N, C, K, B;
   int ntiles, tiles_per_chunk, nchunks;
+  int np;                 // pattern stride of the partials: N, or 64 ntiles for S = 4
   const R* mats;          // [K][B][C][S][S]
   const uint8_t* codes;   // [T][N]
   const R* table;         // [ncodes][S]
   int ncodes;
   const R* dense;         // [T][N][S]
-  R* partials;            // [I][K][N][C][S]; S = 4: [I][K][C][N][4]
-  int32_t* scales;        // [I][K][N]
+  R* partials;            // [I][K][N][C][S]; S = 4: [I][K][C][np][4]
+  int32_t* scales;        // [I][K][N] (S != 4)
+  uint8_t* codes_pad;     // S = 4: codes [T][ntiles * 64], rows 64-byte aligned (same memory)
   const R* freqs;
   int freqs_ks;           // stride between samples (0 or S)
   const R* props;
@@ -43,9 +45,10 @@ int launch_pre_wide(const PassArgs<R>& a, const int32_t* nodes, int width, int t
 // pre-order adjoint over `dna_pre_levels` tree levels per launch. Partials of
 // S = 4 use the category-major layout [slot][K][C][N][4].
 constexpr size_t kDnaLdsBudget = 64 * 1024;
+constexpr size_t kDnaWalkLdsMax = 160 * 1024;  // gfx950 LDS per workgroup
 template <typename R>
-int launch_post_dna(const PassArgs<R>& a, const int32_t* prog, int nprog, int need, int tips,
-                    hipStream_t st);
+int launch_post_dna(const PassArgs<R>& a, const int32_t* prog, const int32_t* segtab, int nseg,
+                    int nprog, int need, int tips, hipStream_t st);
 template <typename R>
 int launch_pre_dna(const PassArgs<R>& a, const int32_t* nodes, int width, int tips, bool root,
                    hipStream_t st);

@@ -9,7 +9,7 @@
 // 64 * tile + l, so the P matrices of a wave are uniform (scalar loads, SGPR
 // operands) and every partial access of a wave is one contiguous 1 KiB (FP32)
 // run of the category-major layout
-//   partials [slot][K][C][N][4]   slot = node - T
+//   partials [slot][K][C][np][4]  slot = node - T, np = 64 x tiles
 // The (C, S)-maximum that rescales a pattern and the per-pattern likelihood
 // are exchanged between the C waves through LDS.
 //
@@ -26,6 +26,7 @@
 // and each branch statistic d loglik / d P accumulates in registers over
 // the block's chunk of tiles.
 #include "pruning.hpp"
+#include "walk.hpp"
 
 #include <math.h>
 
@@ -74,6 +75,28 @@ __device__ __forceinline__ V4<R> zero4() {
 
 __device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
+// Read-only data read with wave-uniform addresses (the walk program, P
+// matrices) goes through the constant address space so the compiler always
+// uses scalar loads; those are counted apart from vector memory, so waiting
+// for them never waits for the partials' stores.
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T*;
+template <typename T>
+__device__ __forceinline__ cptr<T> to_const(const T* p) {
+  return (cptr<T>)p;
+}
+
+// Workgroup barrier that orders LDS accesses only. __syncthreads() also
+// releases global memory at workgroup scope, which waits for every vector
+// memory operation in flight (the stores of partials, prefetched loads) at
+// each barrier; the exchanges here go through LDS, and a thread only reads
+// back global data it wrote itself, so that wait is not needed.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __device__ __forceinline__ float fmax_(float a, float b) { return fmaxf(a, b); }
 __device__ __forceinline__ double fmax_(double a, double b) { return fmax(a, b); }
 __device__ __forceinline__ float frexp_(float m, int* e) { return frexpf(m, e); }
@@ -85,7 +108,14 @@ __device__ __forceinline__ double ldexp_(double x, int e) { return ldexp(x, e); 
 template <typename R>
 __device__ __forceinline__ void load_p(const R* __restrict__ mats, int B, int C, int k, int b, int c,
                                        R (&P)[16]) {
-  const R* p = mats + (((size_t)k * B + b) * C + c) * 16;
+  const cptr<R> p = to_const(mats) + (((size_t)k * B + b) * C + c) * 16;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) P[i] = p[i];
+}
+
+// The 4x4 block at p (16 consecutive values, wave-uniform address).
+template <typename R>
+__device__ __forceinline__ void load_p16(cptr<R> p, R (&P)[16]) {
 #pragma unroll
   for (int i = 0; i < 16; ++i) P[i] = p[i];
 }
@@ -160,7 +190,7 @@ __device__ __forceinline__ V4<R> node4(const PassArgs<R>& a, const V4<R>* tbl, i
                                        int n, bool val) {
   if (ch < a.T) return tip4<R, TIPS>(a, tbl, ch, n, val);
   if (!val) return zero4<R>();
-  return ld4(a.partials + (prow(a.T, a.K, a.C, a.N, ch, k, c) + n) * 4);
+  return ld4(a.partials + (prow(a.T, a.K, a.C, a.np, ch, k, c) + n) * 4);
 }
 
 template <typename R, int TIPS>
@@ -231,117 +261,205 @@ __device__ __forceinline__ void acc_outer(R (&g)[16], R w, const V4<R>& u, const
 }  // namespace
 
 // ------------------------------------------------------------------ post-order
-// prog: int4 per internal node in walk order, {v, left, right, slots} with
-// slots = (left slot + 1) | (right slot + 1) << 8 | (output slot) << 16 and
-// slot 0 meaning "tip". Root last. grid = (ntiles, K), block = 64 C.
-template <typename R, int TIPS>
-__global__ __launch_bounds__(1024) void k_post_walk(PassArgs<R> a, const R* __restrict__ mats,
+// A block of W waves walks the whole tree for one 64-pattern tile; wave w
+// handles sample blockIdx.y * W + w and lane l pattern 64 tile + l with all C
+// categories of it, so the rescaling maximum and the root likelihood never
+// leave the lane and the waves never synchronise after staging. The tile's
+// tip codes (T x 64 bytes) are staged in LDS once per block; P of each
+// (branch, category) is wave-uniform (scalar loads, SGPR operands).
+//
+// prog: kWalkInts ints per internal node in walk order (walk.hpp). A child is
+// a tip, the previous entry's result (registers), or an earlier result that
+// waited on the stack: its partial is read back from the copy this lane
+// wrote to HBM (in L2 in practice), issued one node ahead and before that
+// node's stores so the wait for it does not wait for them, and its exponent
+// comes from the lane's LDS stack. grid = (ntiles, ceil(K / W)), block = 64 W.
+// CM: categories rounded up to a power of two (EXACT: C == CM).
+template <typename R, int TIPS, int CM, bool EXACT>
+__global__ __launch_bounds__(CM >= 8 ? 256 : 1024) void k_post_walk(PassArgs<R> a, const R* __restrict__ mats,
                                                     const int4* __restrict__ prog, int nprog,
-                                                    int dl) {
+                                                    int ne, int W) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int C = a.C, nt = C * 64;
-  const int c = uni(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const int tile = blockIdx.x, k = blockIdx.y;
+  const int C = EXACT ? CM : a.C;
+  const int wave = uni(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int tile = blockIdx.x, k = blockIdx.y * W + wave;
   const int n = tile * 64 + lane;
   const bool val = n < a.N;
-  V4<R>* stk = reinterpret_cast<V4<R>*>(smem_raw);              // [dl][nt]
-  int* est = reinterpret_cast<int*>(stk + (size_t)dl * nt);     // [dl][nt]
-  R* mx = reinterpret_cast<R*>(est + (size_t)dl * nt);          // [2][nt]
-  double* red = reinterpret_cast<double*>(mx + 2 * nt);         // [C][6]
-  V4<R>* tbl = reinterpret_cast<V4<R>*>(red + 6 * C);           // [ncodes]
+  uint8_t* lc = smem_raw;                                              // [T][64]
+  const size_t lc_bytes = TIPS == HX_TIPS_CODES ? (size_t)a.T * 64 : 0;
+  V4<R>* tbl = reinterpret_cast<V4<R>*>(smem_raw + lc_bytes);          // [ncodes]
+  int* est = reinterpret_cast<int*>(tbl + (TIPS == HX_TIPS_CODES ? a.ncodes : 0)) +
+             (size_t)wave * ne * 64;                                   // [W][ne][64]
+  const int nt = blockDim.x;
+  if constexpr (TIPS == HX_TIPS_CODES) {
+    const size_t stride = (size_t)a.ntiles * 64;
+    for (int i = threadIdx.x; i < a.T * 4; i += nt) {
+      const int row = i >> 2, q = i & 3;
+      *reinterpret_cast<uint4*>(lc + row * 64 + q * 16) =
+          *reinterpret_cast<const uint4*>(a.codes_pad + row * stride + tile * 64 + q * 16);
+    }
+  }
   stage_tbl<R, TIPS>(a, tbl, nt);
   __syncthreads();
+  if (k >= a.K) return;
 
-  int par = 0;
-  V4<R> y;
-  int E = 0;
-  for (int i = 0; i < nprog; ++i) {
-    const int4 ins = prog[i];
-    const int v = ins.x, l = ins.y, r = ins.z;
-    const int sl = (ins.w & 0xff) - 1, sr = ((ins.w >> 8) & 0xff) - 1, so = ins.w >> 16;
-    V4<R> xl, xr;
-    int El = 0, Er = 0;
-    if (sl < 0) {
-      xl = tip4<R, TIPS>(a, tbl, l, n, val);
-    } else if (sl < dl) {
-      xl = stk[sl * nt + threadIdx.x];
-      El = est[sl * nt + threadIdx.x];
-    } else {
-      xl = node4<R, TIPS>(a, tbl, l, k, c, n, val);
-      El = val ? a.scales[((size_t)(l - a.T) * a.K + k) * a.N + n] : 0;
-    }
-    if (sr < 0) {
-      xr = tip4<R, TIPS>(a, tbl, r, n, val);
-    } else if (sr < dl) {
-      xr = stk[sr * nt + threadIdx.x];
-      Er = est[sr * nt + threadIdx.x];
-    } else {
-      xr = node4<R, TIPS>(a, tbl, r, k, c, n, val);
-      Er = val ? a.scales[((size_t)(r - a.T) * a.K + k) * a.N + n] : 0;
-    }
-    R P[16];
-    load_p(mats, a.B, C, k, l, c, P);
-    const V4<R> yl = mv(P, xl);
-    load_p(mats, a.B, C, k, r, c, P);
-    const V4<R> yr = mv(P, xr);
+  // element offsets: node slot s, category c of this lane's pattern at
+  // base + s * sstride + c * cstride (layout [slot][K][C][np][4], np = 64 ntiles,
+  // so the padding lanes of the last tile have slots of their own)
+  const size_t cstride = (size_t)a.np * 4;
+  const size_t sstride = (size_t)a.K * C * cstride;
+  R* base = a.partials + (size_t)k * C * cstride + (size_t)n * 4;
+  const cptr<R> mk = to_const(mats) + (size_t)k * a.B * C * 16;
+  const cptr<int> cprog = to_const(reinterpret_cast<const int*>(prog));
+  auto entry = [&](int i, int4& f0, int4& f1) {
+    const cptr<int> q = cprog + (size_t)kWalkInts * i;
+    f0.x = q[0], f0.y = q[1], f0.z = q[2], f0.w = q[3];
+    f1.x = q[4], f1.y = q[5], f1.z = q[6], f1.w = q[7];
+  };
+
+  auto tip = [&](int ch) -> V4<R> {
+    if constexpr (TIPS == HX_TIPS_CODES) return tbl[lc[ch * 64 + lane]];
+    else return val ? ld4(a.dense + ((size_t)ch * a.N + n) * 4) : zero4<R>();
+  };
+  // Issued every node, for the stacked child of the next entry or, when it
+  // has none, for slot 0 (a valid address, value unused): every iteration
+  // then issues the same vector memory operations, so the wait for these
+  // loads counts only the older ones and not this node's stores.
+  auto stacked_of = [&](const int4& f0, const int4& f1) {
+    if ((f1.z & 0xff) == 2) return f0.y;
+    if (((f1.z >> 8) & 0xff) == 2) return f0.z;
+    return a.T;
+  };
+  auto readback = [&](int ch, V4<R>(&x)[CM]) {
+    const R* p = base + (size_t)(ch - a.T) * sstride;
 #pragma unroll
-    for (int s = 0; s < 4; ++s) y.x[s] = yl.x[s] * yr.x[s];
-    mx[par * nt + threadIdx.x] = max4(y);
-    __syncthreads();
+    for (int c = 0; c < CM; ++c)
+      if (EXACT || c < C) x[c] = ld4(p + (size_t)c * cstride);
+  };
+
+  // The first entry of a walk has two tip children, so xs is first read
+  // after a loop iteration has issued its read-back and stores.
+  V4<R> y[CM], xs[CM];
+  int E = 0;
+  int4 e0, e1;
+  entry(0, e0, e1);
+  // entries 0 .. nprog-2 store their partial; the last entry is the root
+  for (int i = 0; i < nprog; ++i) {
+    const bool last = i + 1 == nprog;
+    const int v = e0.x, l = e0.y, r = e0.z;
+    const int sl = (e0.w & 0xff) - 1, sr = ((e0.w >> 8) & 0xff) - 1, so = e0.w >> 16;
+    const int kl = e1.z & 0xff, kr = (e1.z >> 8) & 0xff;
+    int4 f0 = e0, f1 = e1;
+    if (i + 1 < nprog) entry(i + 1, f0, f1);
+    V4<R> xl[CM], xr[CM];
+    int El = 0, Er = 0;
+    if (kl == 0) {
+      const V4<R> t = tip(l);
+#pragma unroll
+      for (int c = 0; c < CM; ++c) xl[c] = t;
+    } else if (kl == 1) {
+#pragma unroll
+      for (int c = 0; c < CM; ++c) xl[c] = y[c];
+      El = E;
+    } else {
+#pragma unroll
+      for (int c = 0; c < CM; ++c) xl[c] = xs[c];
+      El = est[sl * 64 + lane];
+    }
+    if (kr == 0) {
+      const V4<R> t = tip(r);
+#pragma unroll
+      for (int c = 0; c < CM; ++c) xr[c] = t;
+    } else if (kr == 1) {
+#pragma unroll
+      for (int c = 0; c < CM; ++c) xr[c] = y[c];
+      Er = E;
+    } else {
+#pragma unroll
+      for (int c = 0; c < CM; ++c) xr[c] = xs[c];
+      Er = est[sr * 64 + lane];
+    }
+    // P of (l, c) and (r, c); the loads of category c + 1 are issued before
+    // category c is computed
     R M = R(0);
-    for (int cc = 0; cc < C; ++cc) M = fmax_(M, mx[par * nt + cc * 64 + lane]);
-    par ^= 1;
-    const int e = scale_exp(M);
-    scale4(y, e);
-    E = El + Er + e;
-    if (i + 1 < nprog) {
-      if (val) st4(a.partials + (prow(a.T, a.K, C, a.N, v, k, c) + n) * 4, y);
-      if (so < dl) {
-        stk[so * nt + threadIdx.x] = y;
-        est[so * nt + threadIdx.x] = E;
-      } else if (c == 0 && val) {
-        a.scales[((size_t)(v - a.T) * a.K + k) * a.N + n] = E;
+    R Pl[2][16], Pr[2][16];
+    load_p16(mk + (size_t)l * C * 16, Pl[0]);
+    load_p16(mk + (size_t)r * C * 16, Pr[0]);
+#pragma unroll
+    for (int c = 0; c < CM; ++c) {
+      if (EXACT || c < C) {
+        if (c + 1 < CM && (EXACT || c + 1 < C)) {
+          load_p16(mk + ((size_t)l * C + c + 1) * 16, Pl[(c + 1) & 1]);
+          load_p16(mk + ((size_t)r * C + c + 1) * 16, Pr[(c + 1) & 1]);
+        }
+        const V4<R> yl = mv(Pl[c & 1], xl[c]);
+        const V4<R> yr = mv(Pr[c & 1], xr[c]);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) y[c].x[s] = yl.x[s] * yr.x[s];
+        M = fmax_(M, max4(y[c]));
+      } else {
+        y[c] = zero4<R>();
       }
     }
+    const int e = scale_exp(M);
+#pragma unroll
+    for (int c = 0; c < CM; ++c) scale4(y[c], e);
+    E = El + Er + e;
+    if (last) break;
+    // the next entry's stacked child, before this node's stores
+    readback(stacked_of(f0, f1), xs);
+    R* p = base + (size_t)(v - a.T) * sstride;
+#pragma unroll
+    for (int c = 0; c < CM; ++c)
+      if (EXACT || c < C) st4(p + (size_t)c * cstride, y[c]);
+    est[so * 64 + lane] = E;
+    e0 = f0;
+    e1 = f1;
   }
 
   // root: site likelihood sum_c p_c pi.x_c (tree_likelihood.py:71-74, 215-220)
-  const R pc = a.props[(size_t)k * a.props_ks + c];
   R pi[4];
 #pragma unroll
   for (int s = 0; s < 4; ++s) pi[s] = a.freqs[(size_t)k * a.freqs_ks + s];
-  R px = R(0);
-#pragma unroll
-  for (int s = 0; s < 4; ++s) px = fma(pi[s], y.x[s], px);
-  mx[par * nt + threadIdx.x] = pc * px;
-  __syncthreads();
+  R px[CM];
   R lik = R(0);
-  for (int cc = 0; cc < C; ++cc) lik += mx[par * nt + cc * 64 + lane];
-  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // loglik, dfreqs[4], dprops[c]
+#pragma unroll
+  for (int c = 0; c < CM; ++c) {
+    px[c] = R(0);
+    if (c < C) {
+#pragma unroll
+      for (int s = 0; s < 4; ++s) px[c] = fma(pi[s], y[c].x[s], px[c]);
+      lik += a.props[(size_t)k * a.props_ks + c] * px[c];
+    }
+  }
+  double acc[5 + CM];  // loglik, dfreqs[4], dprops[c]
+#pragma unroll
+  for (int q = 0; q < 5 + CM; ++q) acc[q] = 0.0;
   if (val) {
     const double w = a.weights[n];
     const double dl_ = (double)lik;
     const double wi = w / dl_;
-    if (c == 0) acc[0] = w * (log(dl_) + (double)E * kLn2);
+    acc[0] = w * (log(dl_) + (double)E * kLn2);
 #pragma unroll
-    for (int s = 0; s < 4; ++s) acc[1 + s] = wi * (double)(pc * y.x[s]);
-    acc[5] = wi * (double)px;
+    for (int c = 0; c < CM; ++c) {
+      if (c < C) {
+        const R pc = a.props[(size_t)k * a.props_ks + c];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) acc[1 + s] += wi * (double)(pc * y[c].x[s]);
+        acc[5 + c] = wi * (double)px[c];
+      }
+    }
   }
 #pragma unroll
-  for (int q = 0; q < 6; ++q)
+  for (int q = 0; q < 5 + CM; ++q)
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) acc[q] += __shfl_xor(acc[q], o);
-  if (lane == 0)
-#pragma unroll
-    for (int q = 0; q < 6; ++q) red[c * 6 + q] = acc[q];
-  __syncthreads();
   double* out = a.rootsums + ((size_t)k * a.ntiles + tile) * (5 + C);
-  if (threadIdx.x < 5) {
-    double t = 0.0;
-    for (int cc = 0; cc < C; ++cc) t += red[cc * 6 + threadIdx.x];
-    out[threadIdx.x] = t;
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < 5 + CM; ++q)
+      if (q < 5 + C) out[q] = acc[q];
   }
-  if (threadIdx.x < C) out[5 + threadIdx.x] = red[threadIdx.x * 6 + 5];
 }
 
 // ------------------------------------------------------------------- pre-order
@@ -352,21 +470,40 @@ __global__ __launch_bounds__(1024) void k_post_walk(PassArgs<R> a, const R* __re
 //   U_a  = P_a^T Ut_a, rescaled (scale-free: every term is a ratio)
 // LV = 2 also applies this to the children's children, with the children's U
 // kept in registers; only the U of the last level is written (over L, in place).
-// NTMAX: block-size bound (64 C rounded up), which sets the register budget.
-template <typename R, int TIPS, bool ROOT, int LV, int NTMAX>
-__global__ __launch_bounds__(NTMAX) void k_pre_grp(
+// Every global load of a tile (U, the children's and grandchildren's L or tip
+// codes) is issued one tile ahead, before the barriers of the current tile.
+// CM: C rounded up to a power of two (block-size bound and unrolled category
+// loops).
+template <typename R, int LV>
+struct PreTile {
+  V4<R> U;
+  V4<R> Lc[2];
+  V4<R> Lg[LV == 2 ? 2 : 1][2];
+  int cc[2];
+  int cg[LV == 2 ? 2 : 1][2];
+};
+
+template <typename R, int TIPS, bool ROOT, int LV, int CM>
+__global__ __launch_bounds__(64 * CM) void k_pre_grp(
     PassArgs<R> a, const R* __restrict__ mats, const R* __restrict__ mats_dt,
-    const int32_t* __restrict__ nodes) {
+    const int32_t* __restrict__ nodes, int wslots) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
+  constexpr int NQ = LV == 2 ? 4 : 2;  // values exchanged at once
   const int C = a.C, nt = C * 64;
   const int c = uni(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int chunk = blockIdx.x, k = blockIdx.y;
   const int p = nodes[blockIdx.z];
-  R* xs = reinterpret_cast<R*>(smem_raw);                 // [2][4][nt] exchange
-  R* wbuf = xs + 8 * nt;                                  // [C][16][64] reductions
-  V4<R>* tbl = reinterpret_cast<V4<R>*>(wbuf + (size_t)C * 1024);
+  constexpr int NX = 64 * CM;  // exchange rows: unused categories hold 0
+  R* xs = reinterpret_cast<R*>(smem_raw);                 // [2][NQ][NX] exchange
+  R* wbuf = xs + 2 * NQ * NX;                             // [wslots][16][64] reductions
+  V4<R>* tbl = reinterpret_cast<V4<R>*>(wbuf + (size_t)wslots * 1024);
   stage_tbl<R, TIPS>(a, tbl, nt);
-  __syncthreads();
+  if constexpr (CM > 1) {
+    for (int i = threadIdx.x; i < 2 * NQ * (NX - nt); i += nt) {
+      const int row = i / (NX - nt), col = i % (NX - nt);
+      xs[row * NX + nt + col] = R(0);
+    }
+  }
 
   int ch[2], gk[2][2];
   bool in_c[2], in_g[2][2];
@@ -395,21 +532,59 @@ __global__ __launch_bounds__(NTMAX) void k_pre_grp(
     for (int h = 0; h < (LV == 2 ? 2 : 1); ++h) gg[h][0][i] = gg[h][1][i] = R(0);
   }
 
+  // node `nd` (internal: its partial, tip: its code or dense row) at pattern n
+  auto fetch_node = [&](int nd, int n, bool val, V4<R>& L, int& code) {
+    L = zero4<R>();
+    code = 0;
+    if (!val) return;
+    if (nd >= a.T) {
+      L = ld4(a.partials + (prow(a.T, a.K, C, a.np, nd, k, c) + n) * 4);
+    } else if constexpr (TIPS == HX_TIPS_CODES) {
+      code = a.codes[(size_t)nd * a.N + n];
+    } else {
+      L = ld4(a.dense + ((size_t)nd * a.N + n) * 4);
+    }
+  };
+  auto fetch = [&](int tile, PreTile<R, LV>& in) {
+    const int n = tile * 64 + lane;
+    const bool val = n < a.N;
+    in.U = zero4<R>();
+    if constexpr (!ROOT) {
+      if (val) in.U = ld4(a.partials + (prow(a.T, a.K, C, a.np, p, k, c) + n) * 4);
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) fetch_node(ch[h], n, val, in.Lc[h], in.cc[h]);
+    if constexpr (LV == 2) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        if (in_c[h])
+#pragma unroll
+          for (int j = 0; j < 2; ++j) fetch_node(gk[h][j], n, val, in.Lg[h][j], in.cg[h][j]);
+    }
+  };
+  // tip rows come from the LDS table once the codes have arrived
+  auto resolve = [&](int nd, V4<R>& L, int code) {
+    if constexpr (TIPS == HX_TIPS_CODES) {
+      if (nd < a.T) L = tbl[code];
+    }
+  };
+
   int par = 0;
-  // Exchange NQ values per lane with the other category waves of the block.
+  // Exchange up to four values per lane with the other category waves.
   auto xsum = [&](R* vals, int nq, bool take_max) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
-      if (q < nq) xs[(par * 4 + q) * nt + threadIdx.x] = vals[q];
-    __syncthreads();
+    for (int q = 0; q < NQ; ++q)
+      if (q < nq) xs[(par * NQ + q) * NX + threadIdx.x] = vals[q];
+    lds_barrier();
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < NQ; ++q) {
       if (q < nq) {
-        R t = R(0);
-        for (int cc = 0; cc < C; ++cc) {
-          const R u = xs[(par * 4 + q) * nt + cc * 64 + lane];
-          t = take_max ? fmax_(t, u) : t + u;
-        }
+        R u[CM];
+#pragma unroll
+        for (int cc = 0; cc < CM; ++cc) u[cc] = xs[(par * NQ + q) * NX + cc * 64 + lane];
+        R t = u[0];
+#pragma unroll
+        for (int cc = 1; cc < CM; ++cc) t = take_max ? fmax_(t, u[cc]) : t + u[cc];
         vals[q] = t;
       }
     }
@@ -418,23 +593,24 @@ __global__ __launch_bounds__(NTMAX) void k_pre_grp(
 
   const int t0 = chunk * a.tiles_per_chunk;
   const int t1 = min(t0 + a.tiles_per_chunk, a.ntiles);
+  PreTile<R, LV> nxt;
+  if (t0 < t1) fetch(t0, nxt);
+  lds_barrier();  // code table staged
   for (int tile = t0; tile < t1; ++tile) {
+    PreTile<R, LV> cur = nxt;
+    if (tile + 1 < t1) fetch(tile + 1, nxt);
     const int n = tile * 64 + lane;
     const bool val = n < a.N;
     const R w = val ? (R)a.weights[n] : R(0);
-    V4<R> U;
-    if constexpr (ROOT) {
-      U = val ? pi : zero4<R>();
-    } else {
-      U = val ? ld4(a.partials + (prow(a.T, a.K, C, a.N, p, k, c) + n) * 4) : zero4<R>();
-    }
-    V4<R> L[2], PL[2], Ut[2];
+    V4<R> U = cur.U;
+    if constexpr (ROOT) U = val ? pi : zero4<R>();
+    V4<R> PL[2], Ut[2];
     R P[16];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      L[h] = node4<R, TIPS>(a, tbl, ch[h], k, c, n, val);
+      resolve(ch[h], cur.Lc[h], cur.cc[h]);
       load_p(mats, a.B, C, k, ch[h], c, P);
-      PL[h] = mv(P, L[h]);
+      PL[h] = mv(P, cur.Lc[h]);
     }
     R d[4];
     d[0] = R(0);
@@ -447,8 +623,8 @@ __global__ __launch_bounds__(NTMAX) void k_pre_grp(
     d[0] *= pc;
     xsum(d, 1, false);
     const R coef = val ? w / d[0] * pc : R(0);
-    acc_outer(g[0], coef, Ut[0], L[0]);
-    acc_outer(g[1], coef, Ut[1], L[1]);
+    acc_outer(g[0], coef, Ut[0], cur.Lc[0]);
+    acc_outer(g[1], coef, Ut[1], cur.Lc[1]);
     // children's pre-order partials
     V4<R> Uc[2];
     R m[4];
@@ -468,23 +644,23 @@ __global__ __launch_bounds__(NTMAX) void k_pre_grp(
         if (in_c[h]) {
           scale4(Uc[h], scale_exp(m[h]));
           if constexpr (LV == 1) {
-            if (val) st4(a.partials + (prow(a.T, a.K, C, a.N, ch[h], k, c) + n) * 4, Uc[h]);
+            if (val) st4(a.partials + (prow(a.T, a.K, C, a.np, ch[h], k, c) + n) * 4, Uc[h]);
           }
         }
       }
     }
     if constexpr (LV == 2) {
       if (in_c[0] || in_c[1]) {
-        V4<R> Lg[2][2], PLg[2][2], Utg[2][2];
+        V4<R> PLg[2][2], Utg[2][2];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           d[h] = R(0);
           if (in_c[h]) {
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
-              Lg[h][j] = node4<R, TIPS>(a, tbl, gk[h][j], k, c, n, val);
+              resolve(gk[h][j], cur.Lg[h][j], cur.cg[h][j]);
               load_p(mats, a.B, C, k, gk[h][j], c, P);
-              PLg[h][j] = mv(P, Lg[h][j]);
+              PLg[h][j] = mv(P, cur.Lg[h][j]);
             }
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
@@ -501,8 +677,8 @@ __global__ __launch_bounds__(NTMAX) void k_pre_grp(
         for (int h = 0; h < 2; ++h) {
           if (in_c[h]) {
             const R ch_coef = val ? w / d[h] * pc : R(0);
-            acc_outer(gg[h][0], ch_coef, Utg[h][0], Lg[h][0]);
-            acc_outer(gg[h][1], ch_coef, Utg[h][1], Lg[h][1]);
+            acc_outer(gg[h][0], ch_coef, Utg[h][0], cur.Lg[h][0]);
+            acc_outer(gg[h][1], ch_coef, Utg[h][1], cur.Lg[h][1]);
           }
 #pragma unroll
           for (int j = 0; j < 2; ++j) {
@@ -524,22 +700,30 @@ __global__ __launch_bounds__(NTMAX) void k_pre_grp(
               if (in_g[h][j]) {
                 scale4(Utg[h][j], scale_exp(m[2 * h + j]));
                 if (val)
-                  st4(a.partials + (prow(a.T, a.K, C, a.N, gk[h][j], k, c) + n) * 4, Utg[h][j]);
+                  st4(a.partials + (prow(a.T, a.K, C, a.np, gk[h][j], k, c) + n) * 4, Utg[h][j]);
               }
         }
       }
     }
   }
 
-  R* buf = wbuf + (size_t)c * 1024;
+  // Per-wave reduction buffers; when fewer than C fit, waves take turns.
+  R* buf = wbuf + (size_t)(c % wslots) * 1024;
+  const int rounds = (C + wslots - 1) / wslots;
+  for (int rd = 0; rd < rounds; ++rd) {
+    if (c / wslots == rd) {
 #pragma unroll
-  for (int h = 0; h < 2; ++h) flush_branch(a, mats_dt, g[h], buf, ch[h], k, c, chunk);
-  if constexpr (LV == 2) {
+      for (int h = 0; h < 2; ++h) flush_branch(a, mats_dt, g[h], buf, ch[h], k, c, chunk);
+      if constexpr (LV == 2) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h)
-      if (in_c[h])
+        for (int h = 0; h < 2; ++h)
+          if (in_c[h])
 #pragma unroll
-        for (int j = 0; j < 2; ++j) flush_branch(a, mats_dt, gg[h][j], buf, gk[h][j], k, c, chunk);
+            for (int j = 0; j < 2; ++j)
+              flush_branch(a, mats_dt, gg[h][j], buf, gk[h][j], k, c, chunk);
+      }
+    }
+    if (rounds > 1) lds_barrier();
   }
 }
 
@@ -548,32 +732,61 @@ static size_t tbl_bytes(int tips, int ncodes, size_t rb) {
   return tips == HX_TIPS_CODES ? (size_t)ncodes * 4 * rb : 0;
 }
 
-template <typename R>
-size_t dna_walk_lds(int C, int dl, int tips, int ncodes) {
-  const size_t nt = (size_t)C * 64, rb = sizeof(R);
-  return (size_t)dl * nt * (4 * rb + 4) + 2 * nt * rb + 6 * (size_t)C * 8 +
-         tbl_bytes(tips, ncodes, rb);
+// Copies tip codes [T][N] into rows of ntiles * 64 bytes (zero padded) so the
+// walk stages a tile's codes with aligned 16-byte loads.
+__global__ void k_pad_codes(const uint8_t* __restrict__ codes, uint8_t* __restrict__ out, int T,
+                            int N, int npad) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)T * npad) return;
+  const int t = (int)(idx / npad), j = (int)(idx % npad);
+  out[idx] = j < N ? codes[(size_t)t * N + j] : 0;
 }
 
 template <typename R>
-int dna_walk_depth(int C, int need, int tips, int ncodes) {
-  int dl = need;
-  while (dl > 0 && dna_walk_lds<R>(C, dl, tips, ncodes) > kDnaLdsBudget) --dl;
-  return dl;
+size_t dna_walk_lds(int T, int W, int need, int tips, int ncodes) {
+  return (tips == HX_TIPS_CODES ? (size_t)T * 64 : 0) + tbl_bytes(tips, ncodes, sizeof(R)) +
+         (size_t)W * need * 64 * 4;
 }
 
 template <typename R>
-int launch_post_dna(const PassArgs<R>& a, const int32_t* prog, int nprog, int need, int tips,
-                    hipStream_t st) {
-  const int dl = dna_walk_depth<R>(a.C, need, tips, a.ncodes);
-  const size_t lds = dna_walk_lds<R>(a.C, dl, tips, a.ncodes);
-  dim3 grid(a.ntiles, a.K);
-  const int threads = a.C * 64;
+int launch_post_dna(const PassArgs<R>& a, const int32_t* prog, const int32_t* segtab, int nseg,
+                    int nprog, int need, int tips, hipStream_t st) {
+  (void)segtab;
+  (void)nseg;
+  // W samples share the staged codes; 8 keeps their P matrices (K B C 64 B
+  // per sample) within L2; many categories need a smaller block (registers)
+  const int wmax = a.C > 4 ? 4 : 8;
+  int W = a.K < wmax ? a.K : wmax;
+  while (W > 1 && dna_walk_lds<R>(a.T, W, need, tips, a.ncodes) > kDnaWalkLdsMax) W >>= 1;
+  const size_t lds = dna_walk_lds<R>(a.T, W, need, tips, a.ncodes);
+  HX_REQUIRE(lds <= kDnaWalkLdsMax, HX_ERR_UNSUPPORTED,
+             "post-order walk: tip codes of the tile exceed LDS (too many taxa)");
+  if (tips == HX_TIPS_CODES) {
+    const size_t npad = (size_t)a.ntiles * 64, total = (size_t)a.T * npad;
+    k_pad_codes<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(a.codes, a.codes_pad, a.T, a.N,
+                                                                 (int)npad);
+  }
+  dim3 grid(a.ntiles, (a.K + W - 1) / W);
   const int4* pg = reinterpret_cast<const int4*>(prog);
-  if (tips == HX_TIPS_CODES)
-    k_post_walk<R, HX_TIPS_CODES><<<grid, threads, lds, st>>>(a, a.mats, pg, nprog, dl);
-  else
-    k_post_walk<R, HX_TIPS_DENSE><<<grid, threads, lds, st>>>(a, a.mats, pg, nprog, dl);
+#define HX_WALK(CMV, EX)                                                                     \
+  do {                                                                                       \
+    auto kern = tips == HX_TIPS_CODES ? k_post_walk<R, HX_TIPS_CODES, CMV, EX>               \
+                                      : k_post_walk<R, HX_TIPS_DENSE, CMV, EX>;              \
+    if (lds > 64 * 1024)                                                                     \
+      HX_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),                    \
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
+    kern<<<grid, 64 * W, lds, st>>>(a, a.mats, pg, nprog, need, W);                          \
+  } while (0)
+  switch (a.C) {
+    case 1: HX_WALK(1, true); break;
+    case 2: HX_WALK(2, true); break;
+    case 3: HX_WALK(4, false); break;
+    case 4: HX_WALK(4, true); break;
+    default:
+      if (a.C <= 8) HX_WALK(8, false);
+      else HX_WALK(16, false);
+  }
+#undef HX_WALK
   HX_HIP_TRY(hipGetLastError());
   return HX_OK;
 }
@@ -587,24 +800,36 @@ template <typename R>
 int launch_pre_dna(const PassArgs<R>& a, const int32_t* nodes, int width, int tips, bool root,
                    hipStream_t st) {
   const size_t nt = (size_t)a.C * 64, rb = sizeof(R);
-  const size_t lds = 8 * nt * rb + (size_t)a.C * 1024 * rb + tbl_bytes(tips, a.ncodes, rb);
+  const int lv = dna_pre_levels<R>(a.C);
+  const size_t nx = (size_t)next_pow2(a.C) * 64;
+  const size_t fixed = 2 * (lv == 2 ? 4 : 2) * nx * rb + tbl_bytes(tips, a.ncodes, rb);
+  int wslots = a.C;
+  while (wslots > 1 && fixed + (size_t)wslots * 1024 * rb > kDnaLdsBudget) --wslots;
+  const size_t lds = fixed + (size_t)wslots * 1024 * rb;
+  HX_REQUIRE(lds <= kDnaLdsBudget, HX_ERR_UNSUPPORTED, "pre-order LDS exceeds the 64 KiB budget");
   dim3 grid(a.nchunks, a.K, width);
   const int threads = a.C * 64;
-  const int lv = dna_pre_levels<R>(a.C);
-#define HX_PRE(TV, RV, LVV, NTV) \
-  k_pre_grp<R, TV, RV, LVV, NTV><<<grid, threads, lds, st>>>(a, a.mats, a.mats_dt, nodes)
-#define HX_PRE_T(LVV, NTV)                                                               \
+#define HX_PRE(TV, RV, LVV, CMV) \
+  k_pre_grp<R, TV, RV, LVV, CMV><<<grid, threads, lds, st>>>(a, a.mats, a.mats_dt, nodes, wslots)
+#define HX_PRE_T(LVV, CMV)                                                               \
   if (tips == HX_TIPS_CODES) {                                                           \
-    if (root) HX_PRE(HX_TIPS_CODES, true, LVV, NTV); else HX_PRE(HX_TIPS_CODES, false, LVV, NTV); \
+    if (root) HX_PRE(HX_TIPS_CODES, true, LVV, CMV); else HX_PRE(HX_TIPS_CODES, false, LVV, CMV); \
   } else {                                                                               \
-    if (root) HX_PRE(HX_TIPS_DENSE, true, LVV, NTV); else HX_PRE(HX_TIPS_DENSE, false, LVV, NTV); \
+    if (root) HX_PRE(HX_TIPS_DENSE, true, LVV, CMV); else HX_PRE(HX_TIPS_DENSE, false, LVV, CMV); \
   }
+  const int cm = next_pow2(a.C);
   if (lv == 2) {
-    if constexpr (std::is_same<R, float>::value) HX_PRE_T(2, 256)
-  } else if (threads <= 512) {
-    HX_PRE_T(1, 512)
+    if constexpr (std::is_same<R, float>::value) {
+      if (cm == 1) { HX_PRE_T(2, 1) } else if (cm == 2) { HX_PRE_T(2, 2) } else { HX_PRE_T(2, 4) }
+    }
   } else {
-    HX_PRE_T(1, 1024)
+    switch (cm) {
+      case 1: HX_PRE_T(1, 1) break;
+      case 2: HX_PRE_T(1, 2) break;
+      case 4: HX_PRE_T(1, 4) break;
+      case 8: HX_PRE_T(1, 8) break;
+      default: HX_PRE_T(1, 16) break;
+    }
   }
 #undef HX_PRE_T
 #undef HX_PRE
@@ -612,10 +837,10 @@ int launch_pre_dna(const PassArgs<R>& a, const int32_t* nodes, int width, int ti
   return HX_OK;
 }
 
-template int launch_post_dna<float>(const PassArgs<float>&, const int32_t*, int, int, int,
-                                    hipStream_t);
-template int launch_post_dna<double>(const PassArgs<double>&, const int32_t*, int, int, int,
-                                     hipStream_t);
+template int launch_post_dna<float>(const PassArgs<float>&, const int32_t*, const int32_t*, int,
+                                    int, int, int, hipStream_t);
+template int launch_post_dna<double>(const PassArgs<double>&, const int32_t*, const int32_t*, int,
+                                     int, int, int, hipStream_t);
 template int launch_pre_dna<float>(const PassArgs<float>&, const int32_t*, int, int, bool,
                                    hipStream_t);
 template int launch_pre_dna<double>(const PassArgs<double>&, const int32_t*, int, int, bool,

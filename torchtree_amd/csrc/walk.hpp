@@ -8,9 +8,21 @@
 //   need(tip) = 0
 //   need(v)   = max(need(a), need(b), 1 + min(need(a), need(b)))  both internal
 //             = max(1, need(a), need(b))                          otherwise
-// which is at most floor(log2(T)) + 1. Each internal node becomes one int4
-// {v, left, right, slots}, slots = (left slot + 1) | (right slot + 1) << 8 |
-// (output slot) << 16 with slot + 1 = 0 for a tip; the root comes last.
+// which is at most floor(log2(T)) + 1. Each internal node becomes one entry
+// of kWalkInts ints, root last:
+//   [0] v  [1] left  [2] right
+//   [3] slots = (left slot + 1) | (right slot + 1) << 8 | (output slot) << 16,
+//       slot + 1 = 0 for a tip child
+//   [4] tpos  = (left tip position + 1) | (right tip position + 1) << 8: the
+//       row of the tip's codes in the current tip segment (0 = not a tip)
+//   [5] seg   = index of the tip segment this entry opens, or -1
+//   [6] kinds = kind(left) | kind(right) << 8 with kind 0 = tip, 1 = the
+//       result of the previous entry, 2 = an earlier result (stacked: the
+//       first-evaluated child of a node whose children are both internal)
+// Tips are consumed in walk order; consecutive entries are grouped into
+// segments of at most kWalkSegTips tips so a kernel can stage the codes of
+// one segment while it works through the previous one. segtab holds
+// kWalkSegTips tip ids per segment (-1 padding).
 #pragma once
 
 #include <algorithm>
@@ -19,10 +31,13 @@
 
 namespace hx {
 
+constexpr int kWalkInts = 8;
+constexpr int kWalkSegTips = 16;
+
 // kids: [2 * nn] children of each node (tips 0..T-1 have -1); postorder:
 // [I][3] triples, root last. Returns false if the program is malformed.
 inline bool build_walk(int T, const std::vector<int32_t>& kids, const int32_t* postorder,
-                       std::vector<int32_t>& walk, int& walk_need) {
+                       std::vector<int32_t>& walk, std::vector<int32_t>& segtab, int& walk_need) {
   const int I = T - 1, nn = 2 * T - 1;
   const int root = postorder[3 * (I - 1)];
   std::vector<int> need(nn, 0);
@@ -36,7 +51,9 @@ inline bool build_walk(int T, const std::vector<int32_t>& kids, const int32_t* p
     int v, base, state;
   };
   walk.clear();
-  walk.reserve(4 * (size_t)I);
+  walk.reserve(kWalkInts * (size_t)I);
+  segtab.clear();
+  int seg_fill = kWalkSegTips;  // forces a segment at the first entry
   walk_need = 0;
   std::vector<Frame> fs;
   fs.push_back({root, 0, 0});
@@ -68,15 +85,39 @@ inline bool build_walk(int T, const std::vector<int32_t>& kids, const int32_t* p
     }
     const int sa = a < T ? -1 : (a == first ? f.base : f.base + 1);
     const int sb = b < T ? -1 : (b == first ? f.base : f.base + 1);
+    const int ntips = (a < T) + (b < T);
+    int seg = -1;
+    if (seg_fill + ntips > kWalkSegTips) {
+      seg = (int)(segtab.size() / kWalkSegTips);
+      segtab.insert(segtab.end(), kWalkSegTips, -1);
+      seg_fill = 0;
+    }
+    const size_t row0 = segtab.size() - kWalkSegTips;
+    int pa = -1, pb = -1;
+    if (a < T) {
+      pa = seg_fill++;
+      segtab[row0 + pa] = a;
+    }
+    if (b < T) {
+      pb = seg_fill++;
+      segtab[row0 + pb] = b;
+    }
+    const int prev = walk.empty() ? -1 : walk[walk.size() - kWalkInts];
+    const int ka = a < T ? 0 : (a == prev ? 1 : 2);
+    const int kb = b < T ? 0 : (b == prev ? 1 : 2);
     walk.push_back(f.v);
     walk.push_back(a);
     walk.push_back(b);
     walk.push_back((sa + 1) | ((sb + 1) << 8) | (f.base << 16));
+    walk.push_back((pa + 1) | ((pb + 1) << 8));
+    walk.push_back(seg);
+    walk.push_back(ka | (kb << 8));
+    walk.push_back(0);
     walk_need = std::max(walk_need, f.base + 1 + (second >= 0 ? 1 : 0));
     fs.pop_back();
   }
-  return (int)walk.size() == 4 * I && walk[4 * (I - 1)] == root && walk_need <= 255 &&
-         walk_need == need[root];
+  return (int)walk.size() == kWalkInts * I && walk[kWalkInts * (I - 1)] == root &&
+         walk_need <= 255 && walk_need == need[root] && walk[5] == 0;
 }
 
 }  // namespace hx
