@@ -64,18 +64,6 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def even_depth_internal(post, T):
-    """Internal non-root nodes at even depth >= 2: the pre-order group roots
-    whose U is written and read back by the two-level pre-order kernel."""
-    kids = {v: (a, b) for v, a, b in post}
-    root = post[-1][0]
-    depth = {root: 0}
-    for v, a, b in reversed(post):
-        depth[a] = depth[v] + 1
-        depth[b] = depth[v] + 1
-    return sum(1 for v in kids if v != root and depth[v] % 2 == 0)
-
-
 def walk_need(post, T):
     """Stack entries of the post-order walk (larger-need child first)."""
     need = {}
@@ -85,14 +73,15 @@ def walk_need(post, T):
     return need[post[-1][0]]
 
 
-def algorithmic_bytes(post, T, N, C, K, S, rb, nchunks, pre_levels):
+def algorithmic_bytes(T, N, C, K, S, rb, nchunks):
     """Bytes each pass must move at minimum with this algorithm (one launch set).
 
-    P = K N C S rb is one node's partials. Post-order walk: every internal
-    non-root partial written once, nothing read back, tip codes once. Pre-order
-    over `pre_levels` levels per launch: every internal non-root L read once;
-    with two levels only the group roots (even depth >= 2) have U written and
-    read back, with one level every internal non-root U is.
+    P = K N C S rb is one node's partials. Four states: the post-order walk
+    writes every internal non-root partial once and reads none back from HBM
+    (a child that waited on the stack is re-read from L2), plus the tip codes;
+    the pre-order reads each internal non-root node's L and its parent's U and
+    writes its U (3 P per node). Other state counts use level kernels: post
+    2 P per node (written, read by the parent) plus the exponents.
     """
     part = K * N * C * S * rb
     I = T - 1
@@ -100,13 +89,11 @@ def algorithmic_bytes(post, T, N, C, K, S, rb, nchunks, pre_levels):
     tips = T * N * 1
     weights = N * 8
     gpart = (2 * T - 2) * K * nchunks * C * S * S * rb
-    if S != 4:  # dense-state level kernels: written once, read once by the parent
+    if S == 4:
+        post_b = (I - 1) * part + tips + mats + weights
+    else:
         post_b = 2 * (I - 1) * part + 2 * (I - 1) * K * N * 4 + tips + mats + weights
-        pre_b = 3 * (I - 1) * part + tips + mats + weights + gpart
-        return post_b, pre_b
-    post_b = (I - 1) * part + tips + mats + weights
-    nu = even_depth_internal(post, T) if pre_levels == 2 else I - 1
-    pre_b = (I - 1) * part + 2 * nu * part + tips + mats + weights + gpart
+    pre_b = 3 * (I - 1) * part + tips + mats + weights + gpart
     return post_b, pre_b
 
 
@@ -187,8 +174,7 @@ def main():
     ll32 = ll32.detach().cpu()
     g32 = (b32.grad.detach().cpu(), e32.grad.detach().cpu(), f32.grad.detach().cpu())
     nchunks = _nchunks(wl.N, wl.C, wl.K)
-    pre_levels = 2 if (S == 4 and wl.C <= 4) else 1
-    post_b, pre_b = algorithmic_bytes(post, wl.T, wl.N, wl.C, wl.K, S, 4, nchunks, pre_levels)
+    post_b, pre_b = algorithmic_bytes(wl.T, wl.N, wl.C, wl.K, S, 4, nchunks)
     post_ms = phases["postorder"]["ms"]
     pre_ms = phases["preorder"]["ms"]
     post_gbs = post_b / (post_ms * 1e-3) / 1e9
@@ -223,7 +209,7 @@ def main():
         "phases_ms": {k: v["ms"] for k, v in phases.items()},
         "roofline": {
             "bound": "hbm",
-            "kernel": (f"k_pre_grp ({launches - 1} launches, {pre_levels} tree levels each) + k_reduce_g"
+            "kernel": (f"k_pre ({launches - 1} depth-level launches) + k_reduce_g"
                        if dom == "preorder" else "k_post_walk (one launch, whole tree) + k_reduce_root"),
             "achieved": dom_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dom_gbs / HBM_PEAK_GBS,
             "traffic": None, "algorithmic_bytes": dom_b, "pass_ms": dom_ms,

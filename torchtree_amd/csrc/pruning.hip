@@ -236,8 +236,7 @@ static int run_pre(const CallArgs& c, double* out, hipStream_t st) {
   const PassArgs<R> a = make_args<R>(c);
   const hx_plan_s* plan = c.plan;
   const int D = (int)plan->pre_off.size() - 1;
-  const int step = c.S == 4 ? dna_pre_levels<R>(c.C) : 1;
-  for (int d = 0; d < D; d += step) {
+  for (int d = 0; d < D; ++d) {
     const int off = plan->pre_off[d], width = plan->pre_off[d + 1] - off;
     const int rc = c.S == 4
                        ? launch_pre_dna<R>(a, plan->d_pre + off, width, c.tip_kind, d == 0, st)

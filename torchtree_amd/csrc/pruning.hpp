@@ -42,8 +42,8 @@ int launch_pre_wide(const PassArgs<R>& a, const int32_t* nodes, int width, int t
                     int S, hipStream_t st);
 
 // Four-state kernels (pruning_dna.hip): fused post-order walk and the
-// pre-order adjoint over `dna_pre_levels` tree levels per launch. Partials of
-// S = 4 use the category-major layout [slot][K][C][N][4].
+// pre-order adjoint one level per launch. Partials of S = 4 use the
+// category-major layout [slot][K][C][np][4].
 constexpr size_t kDnaLdsBudget = 64 * 1024;
 constexpr size_t kDnaWalkLdsMax = 160 * 1024;  // gfx950 LDS per workgroup
 template <typename R>
@@ -52,8 +52,6 @@ int launch_post_dna(const PassArgs<R>& a, const int32_t* prog, const int32_t* se
 template <typename R>
 int launch_pre_dna(const PassArgs<R>& a, const int32_t* nodes, int width, int tips, bool root,
                    hipStream_t st);
-template <typename R>
-int dna_pre_levels(int C);
 
 // FP32 dense-state kernels on MFMA (pruning_mfma.hip); used when use_mfma().
 bool use_mfma(int C, int S, int dtype);

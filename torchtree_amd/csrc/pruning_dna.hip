@@ -1,30 +1,21 @@
-// Four-state (nucleotide) pruning kernels: one fused post-order walk per
-// pattern tile and a pre-order adjoint that covers two tree levels per launch.
+// Four-state (nucleotide) pruning kernels: one fused post-order walk of the
+// whole tree per pattern tile, and the pre-order adjoint one tree level per
+// launch.
 //
 // Reference: calculate_treelikelihood_discrete* (torchtree/evolution/
 // tree_likelihood.py:40-278) at S = 4, where each node update is two 4x4
 // matrix-vector products per (sample, pattern, category): HBM-bound work.
 //
-// Block = C waves; wave c handles rate category c and lane l pattern
-// 64 * tile + l, so the P matrices of a wave are uniform (scalar loads, SGPR
-// operands) and every partial access of a wave is one contiguous 1 KiB (FP32)
-// run of the category-major layout
+// Partials use the category-major layout
 //   partials [slot][K][C][np][4]  slot = node - T, np = 64 x tiles
-// The (C, S)-maximum that rescales a pattern and the per-pattern likelihood
-// are exchanged between the C waves through LDS.
+// so a wave's accesses to one category are contiguous 16-byte vectors, and
+// the padding lanes of the last tile have slots of their own.
 //
-// Post-order: one block walks the whole tree for its 64-pattern tile. Each
-// internal partial is written to HBM once (the adjoint needs it) and kept on
-// a per-thread LDS stack until its parent consumes it, so no partial is read
-// back from HBM. The walk order (larger-need child first) bounds the stack at
-// about log2(T) entries; entries beyond the LDS depth `dl` are read back from
-// HBM with their exponent from `scales`.
-//
-// Pre-order: a block takes one node p at an even depth and handles the
-// branches to its children and grandchildren. U at the children stays in
-// registers, so only every second level of pre-order partials touches HBM,
-// and each branch statistic d loglik / d P accumulates in registers over
-// the block's chunk of tiles.
+// Post-order (k_post_walk): lane = pattern with all C categories in the lane,
+// one wave per (tile, sample), walking the tree program; each internal
+// partial is written once and never read back except by the lane that wrote
+// it, when a child waited on the stack (an L2 hit). Pre-order (k_pre): thread
+// = (pattern, category), one block per (chunk, sample, parent) of a level.
 #include "pruning.hpp"
 #include "walk.hpp"
 
@@ -86,32 +77,12 @@ __device__ __forceinline__ cptr<T> to_const(const T* p) {
   return (cptr<T>)p;
 }
 
-// Workgroup barrier that orders LDS accesses only. __syncthreads() also
-// releases global memory at workgroup scope, which waits for every vector
-// memory operation in flight (the stores of partials, prefetched loads) at
-// each barrier; the exchanges here go through LDS, and a thread only reads
-// back global data it wrote itself, so that wait is not needed.
-__device__ __forceinline__ void lds_barrier() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-  __builtin_amdgcn_s_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
 __device__ __forceinline__ float fmax_(float a, float b) { return fmaxf(a, b); }
 __device__ __forceinline__ double fmax_(double a, double b) { return fmax(a, b); }
 __device__ __forceinline__ float frexp_(float m, int* e) { return frexpf(m, e); }
 __device__ __forceinline__ double frexp_(double m, int* e) { return frexp(m, e); }
 __device__ __forceinline__ float ldexp_(float x, int e) { return ldexpf(x, e); }
 __device__ __forceinline__ double ldexp_(double x, int e) { return ldexp(x, e); }
-
-// P of (sample k, branch b, category c), a wave-uniform 4x4 block.
-template <typename R>
-__device__ __forceinline__ void load_p(const R* __restrict__ mats, int B, int C, int k, int b, int c,
-                                       R (&P)[16]) {
-  const cptr<R> p = to_const(mats) + (((size_t)k * B + b) * C + c) * 16;
-#pragma unroll
-  for (int i = 0; i < 16; ++i) P[i] = p[i];
-}
 
 // The 4x4 block at p (16 consecutive values, wave-uniform address).
 template <typename R>
@@ -172,78 +143,10 @@ __device__ __forceinline__ size_t prow(int T, int K, int C, int N, int v, int k,
   return (((size_t)(v - T) * K + k) * C + c) * (size_t)N;
 }
 
-// Tip partial of tip `ch` at pattern n (zero for padding lanes).
-template <typename R, int TIPS>
-__device__ __forceinline__ V4<R> tip4(const PassArgs<R>& a, const V4<R>* tbl, int ch, int n,
-                                      bool val) {
-  if (!val) return zero4<R>();
-  if constexpr (TIPS == HX_TIPS_CODES) {
-    return tbl[a.codes[(size_t)ch * a.N + n]];
-  } else {
-    return ld4(a.dense + ((size_t)ch * a.N + n) * 4);
-  }
-}
-
-// Post-order partial of node ch (tip or internal) at pattern n.
-template <typename R, int TIPS>
-__device__ __forceinline__ V4<R> node4(const PassArgs<R>& a, const V4<R>* tbl, int ch, int k, int c,
-                                       int n, bool val) {
-  if (ch < a.T) return tip4<R, TIPS>(a, tbl, ch, n, val);
-  if (!val) return zero4<R>();
-  return ld4(a.partials + (prow(a.T, a.K, a.C, a.np, ch, k, c) + n) * 4);
-}
-
 template <typename R, int TIPS>
 __device__ __forceinline__ void stage_tbl(const PassArgs<R>& a, V4<R>* tbl, int nt) {
   if constexpr (TIPS == HX_TIPS_CODES) {
     for (int i = threadIdx.x; i < a.ncodes; i += nt) tbl[i] = ld4(a.table + 4 * i);
-  }
-}
-
-// Sums 16 per-lane values over the 64 lanes of the calling wave, in double.
-// `buf` is this wave's LDS scratch of 16 x 64 R. Lane 4 v (v = 0..15) returns
-// the total of value v; the other lanes return partial sums.
-template <typename R>
-__device__ __forceinline__ double wave_sum16(const R (&g)[16], R* buf) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int j = 0; j < 16; ++j) buf[j * 64 + lane] = g[j];
-  __builtin_amdgcn_wave_barrier();
-  const int v = lane >> 2, q = lane & 3;
-  const R* row = buf + v * 64 + q * 16;
-  double s = 0.0;
-#pragma unroll
-  for (int i = 0; i < 16; i += 4) {
-    const V4<R> t = ld4(row + i);
-    s += (double)t.x[0];
-    s += (double)t.x[1];
-    s += (double)t.x[2];
-    s += (double)t.x[3];
-  }
-  __builtin_amdgcn_wave_barrier();
-  s += __shfl_xor(s, 1);
-  s += __shfl_xor(s, 2);
-  return s;
-}
-
-// Writes the reduced statistic of branch b (category c) for one chunk: the 16
-// entries of d loglik / d P, or their contraction with dP/dtau.
-template <typename R>
-__device__ __forceinline__ void flush_branch(const PassArgs<R>& a, const R* __restrict__ mats_dt,
-                                             const R (&g)[16], R* buf, int b, int k, int c,
-                                             int chunk) {
-  const int lane = threadIdx.x & 63;
-  const double s = wave_sum16(g, buf);
-  const bool owner = (lane & 3) == 0;
-  const int v = lane >> 2;
-  if (mats_dt != nullptr) {
-    const R* q = mats_dt + (((size_t)k * a.B + b) * a.C + c) * 16;
-    double t = owner ? s * (double)q[v] : 0.0;
-#pragma unroll
-    for (int o = 4; o < 64; o <<= 1) t += __shfl_xor(t, o);
-    if (lane == 0) a.gtau[(((size_t)b * a.K + k) * a.nchunks + chunk) * a.C + c] = t;
-  } else if (owner) {
-    a.gpart[((((size_t)b * a.K + k) * a.nchunks + chunk) * a.C + c) * 16 + v] = (R)s;
   }
 }
 
@@ -463,268 +366,193 @@ __global__ __launch_bounds__(CM >= 8 ? 256 : 1024) void k_post_walk(PassArgs<R> 
 }
 
 // ------------------------------------------------------------------- pre-order
-// Group rooted at p (root: U = freqs). For parent q with children a, b:
-//   Ut_a = U_q * (P_b L_b), Ut_b = U_q * (P_a L_a)
+// For parent p with children a, b and pre-order partial U_p (root: freqs):
+//   Ut_a = U_p * (P_b L_b), Ut_b = U_p * (P_a L_a)
 //   lik  = sum_c p_c Ut_a.(P_a L_a)                    (scaled site likelihood)
 //   d loglik / d P_a[c][i][j] += w p_c Ut_a[i] L_a[j] / lik
-//   U_a  = P_a^T Ut_a, rescaled (scale-free: every term is a ratio)
-// LV = 2 also applies this to the children's children, with the children's U
-// kept in registers; only the U of the last level is written (over L, in place).
-// Every global load of a tile (U, the children's and grandchildren's L or tip
-// codes) is issued one tile ahead, before the barriers of the current tile.
-// CM: C rounded up to a power of two (block-size bound and unrolled category
-// loops).
-template <typename R, int LV>
+//   U_a  = P_a^T Ut_a, rescaled (scale-free: every term is a ratio), written
+//          over L_a in place
+// Thread (pattern, category) with the CP = next_pow2(C) categories of a
+// pattern in adjacent lanes: the likelihood sum and the rescaling maximum are
+// shuffles within those lanes, so the tile loop has no barriers. The two P
+// blocks are staged in LDS once per block; each branch statistic accumulates
+// in registers over the block's chunk of 64-pattern tiles, and every global
+// load of a tile is issued one tile ahead.
+template <typename R>
 struct PreTile {
   V4<R> U;
-  V4<R> Lc[2];
-  V4<R> Lg[LV == 2 ? 2 : 1][2];
-  int cc[2];
-  int cg[LV == 2 ? 2 : 1][2];
+  V4<R> L[2];
+  int code[2];
 };
 
-template <typename R, int TIPS, bool ROOT, int LV, int CM>
-__global__ __launch_bounds__(64 * CM) void k_pre_grp(
-    PassArgs<R> a, const R* __restrict__ mats, const R* __restrict__ mats_dt,
-    const int32_t* __restrict__ nodes, int wslots) {
+template <int CP, typename T>
+__device__ __forceinline__ T qsum(T v) {
+#pragma unroll
+  for (int o = 1; o < CP; o <<= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+template <int CP, typename T>
+__device__ __forceinline__ T qmax(T v) {
+#pragma unroll
+  for (int o = 1; o < CP; o <<= 1) v = fmax_(v, __shfl_xor(v, o));
+  return v;
+}
+
+// Entries of a 16-entry statistic transposed per pass so that the buffer
+// [64][CP][NE + 1] stays within 36 KiB.
+constexpr int pre_pass_entries(int cp, int rb) {
+  int ne = 16;
+  while (ne > 1 && (size_t)64 * cp * (ne + 1) * rb > 36 * 1024) ne >>= 1;
+  return ne;
+}
+
+template <typename R, int TIPS, bool ROOT, int CP>
+__global__ __launch_bounds__(64 * CP) void k_pre(PassArgs<R> a, const R* __restrict__ mats,
+                                                 const R* __restrict__ mats_dt,
+                                                 const int32_t* __restrict__ nodes) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  constexpr int NQ = LV == 2 ? 4 : 2;  // values exchanged at once
-  const int C = a.C, nt = C * 64;
-  const int c = uni(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int C = a.C, nt = 64 * CP;
+  const int t = threadIdx.x, c = t % CP, pl = t / CP;
+  const bool cval = c < C;
+  const int cc = cval ? c : 0;
   const int chunk = blockIdx.x, k = blockIdx.y;
   const int p = nodes[blockIdx.z];
-  constexpr int NX = 64 * CM;  // exchange rows: unused categories hold 0
-  R* xs = reinterpret_cast<R*>(smem_raw);                 // [2][NQ][NX] exchange
-  R* wbuf = xs + 2 * NQ * NX;                             // [wslots][16][64] reductions
-  V4<R>* tbl = reinterpret_cast<V4<R>*>(wbuf + (size_t)wslots * 1024);
-  stage_tbl<R, TIPS>(a, tbl, nt);
-  if constexpr (CM > 1) {
-    for (int i = threadIdx.x; i < 2 * NQ * (NX - nt); i += nt) {
-      const int row = i / (NX - nt), col = i % (NX - nt);
-      xs[row * NX + nt + col] = R(0);
-    }
-  }
-
-  int ch[2], gk[2][2];
-  bool in_c[2], in_g[2][2];
+  int ch[2];
+  bool in_c[2];
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     ch[h] = a.kids[2 * p + h];
     in_c[h] = ch[h] >= a.T;
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      gk[h][j] = (LV == 2 && in_c[h]) ? a.kids[2 * ch[h] + j] : -1;
-      in_g[h][j] = gk[h][j] >= a.T;
-    }
   }
-  const R pc = a.props[(size_t)k * a.props_ks + c];
+  R* Ps = reinterpret_cast<R*>(smem_raw);                         // [2][C][16]
+  V4<R>* tbl = reinterpret_cast<V4<R>*>(Ps + 2 * C * 16);         // [ncodes]
+  // gb: [64 patterns][CP][NE + 1] statistic transpose for the chunk
+  // reduction, NE of the 16 entries per pass
+  constexpr int NE = pre_pass_entries(CP, sizeof(R));
+  R* gb = reinterpret_cast<R*>(tbl + (TIPS == HX_TIPS_CODES ? a.ncodes : 0));
+  for (int i = t; i < 2 * C * 16; i += nt) {
+    const int h = i / (C * 16), rem = i - h * C * 16;
+    Ps[i] = mats[((size_t)k * a.B + ch[h]) * C * 16 + rem];
+  }
+  stage_tbl<R, TIPS>(a, tbl, nt);
+
+  const R pc = cval ? a.props[(size_t)k * a.props_ks + c] : R(0);
   V4<R> pi;
   if constexpr (ROOT) {
 #pragma unroll
     for (int s = 0; s < 4; ++s) pi.x[s] = a.freqs[(size_t)k * a.freqs_ks + s];
   }
   R g[2][16];
-  R gg[LV == 2 ? 2 : 1][2][16];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    g[0][i] = g[1][i] = R(0);
-#pragma unroll
-    for (int h = 0; h < (LV == 2 ? 2 : 1); ++h) gg[h][0][i] = gg[h][1][i] = R(0);
-  }
+  for (int i = 0; i < 16; ++i) g[0][i] = g[1][i] = R(0);
 
-  // node `nd` (internal: its partial, tip: its code or dense row) at pattern n
-  auto fetch_node = [&](int nd, int n, bool val, V4<R>& L, int& code) {
-    L = zero4<R>();
-    code = 0;
-    if (!val) return;
-    if (nd >= a.T) {
-      L = ld4(a.partials + (prow(a.T, a.K, C, a.np, nd, k, c) + n) * 4);
-    } else if constexpr (TIPS == HX_TIPS_CODES) {
-      code = a.codes[(size_t)nd * a.N + n];
-    } else {
-      L = ld4(a.dense + ((size_t)nd * a.N + n) * 4);
-    }
-  };
-  auto fetch = [&](int tile, PreTile<R, LV>& in) {
-    const int n = tile * 64 + lane;
-    const bool val = n < a.N;
+  auto fetch = [&](int tile, PreTile<R>& in) {
+    const int n = tile * 64 + pl;
     in.U = zero4<R>();
     if constexpr (!ROOT) {
-      if (val) in.U = ld4(a.partials + (prow(a.T, a.K, C, a.np, p, k, c) + n) * 4);
+      if (cval) in.U = ld4(a.partials + (prow(a.T, a.K, C, a.np, p, k, cc) + n) * 4);
     }
 #pragma unroll
-    for (int h = 0; h < 2; ++h) fetch_node(ch[h], n, val, in.Lc[h], in.cc[h]);
-    if constexpr (LV == 2) {
-#pragma unroll
-      for (int h = 0; h < 2; ++h)
-        if (in_c[h])
-#pragma unroll
-          for (int j = 0; j < 2; ++j) fetch_node(gk[h][j], n, val, in.Lg[h][j], in.cg[h][j]);
-    }
-  };
-  // tip rows come from the LDS table once the codes have arrived
-  auto resolve = [&](int nd, V4<R>& L, int code) {
-    if constexpr (TIPS == HX_TIPS_CODES) {
-      if (nd < a.T) L = tbl[code];
-    }
-  };
-
-  int par = 0;
-  // Exchange up to four values per lane with the other category waves.
-  auto xsum = [&](R* vals, int nq, bool take_max) {
-#pragma unroll
-    for (int q = 0; q < NQ; ++q)
-      if (q < nq) xs[(par * NQ + q) * NX + threadIdx.x] = vals[q];
-    lds_barrier();
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      if (q < nq) {
-        R u[CM];
-#pragma unroll
-        for (int cc = 0; cc < CM; ++cc) u[cc] = xs[(par * NQ + q) * NX + cc * 64 + lane];
-        R t = u[0];
-#pragma unroll
-        for (int cc = 1; cc < CM; ++cc) t = take_max ? fmax_(t, u[cc]) : t + u[cc];
-        vals[q] = t;
+    for (int h = 0; h < 2; ++h) {
+      in.L[h] = zero4<R>();
+      in.code[h] = 0;
+      if (in_c[h]) {
+        if (cval) in.L[h] = ld4(a.partials + (prow(a.T, a.K, C, a.np, ch[h], k, cc) + n) * 4);
+      } else if constexpr (TIPS == HX_TIPS_CODES) {
+        if (n < a.N) in.code[h] = a.codes[(size_t)ch[h] * a.N + n];
+      } else {
+        if (n < a.N) in.L[h] = ld4(a.dense + ((size_t)ch[h] * a.N + n) * 4);
       }
     }
-    par ^= 1;
+  };
+  auto getP = [&](int h, R(&P)[16]) {
+    const R* src = Ps + (h * C + cc) * 16;
+#pragma unroll
+    for (int i = 0; i < 16; i += 4) {
+      const V4<R> v = ld4(src + i);
+      P[i] = v.x[0], P[i + 1] = v.x[1], P[i + 2] = v.x[2], P[i + 3] = v.x[3];
+    }
   };
 
   const int t0 = chunk * a.tiles_per_chunk;
   const int t1 = min(t0 + a.tiles_per_chunk, a.ntiles);
-  PreTile<R, LV> nxt;
+  PreTile<R> nxt;
   if (t0 < t1) fetch(t0, nxt);
-  lds_barrier();  // code table staged
+  __syncthreads();  // P blocks and code table staged
   for (int tile = t0; tile < t1; ++tile) {
-    PreTile<R, LV> cur = nxt;
+    PreTile<R> cur = nxt;
     if (tile + 1 < t1) fetch(tile + 1, nxt);
-    const int n = tile * 64 + lane;
-    const bool val = n < a.N;
+    const int n = tile * 64 + pl;
+    const bool val = n < a.N && cval;
     const R w = val ? (R)a.weights[n] : R(0);
     V4<R> U = cur.U;
-    if constexpr (ROOT) U = val ? pi : zero4<R>();
+    if constexpr (ROOT) U = cval ? pi : zero4<R>();
     V4<R> PL[2], Ut[2];
     R P[16];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      resolve(ch[h], cur.Lc[h], cur.cc[h]);
-      load_p(mats, a.B, C, k, ch[h], c, P);
-      PL[h] = mv(P, cur.Lc[h]);
+      if constexpr (TIPS == HX_TIPS_CODES) {
+        if (!in_c[h]) cur.L[h] = tbl[cur.code[h]];
+      }
+      getP(h, P);
+      PL[h] = mv(P, cur.L[h]);
     }
-    R d[4];
-    d[0] = R(0);
+    R d = R(0);
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       Ut[0].x[s] = U.x[s] * PL[1].x[s];
       Ut[1].x[s] = U.x[s] * PL[0].x[s];
-      d[0] = fma(Ut[0].x[s], PL[0].x[s], d[0]);
+      d = fma(Ut[0].x[s], PL[0].x[s], d);
     }
-    d[0] *= pc;
-    xsum(d, 1, false);
-    const R coef = val ? w / d[0] * pc : R(0);
-    acc_outer(g[0], coef, Ut[0], cur.Lc[0]);
-    acc_outer(g[1], coef, Ut[1], cur.Lc[1]);
-    // children's pre-order partials
-    V4<R> Uc[2];
-    R m[4];
+    d = qsum<CP>(d * pc);
+    const R coef = val ? w / d * pc : R(0);
+    acc_outer(g[0], coef, Ut[0], cur.L[0]);
+    acc_outer(g[1], coef, Ut[1], cur.L[1]);
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      m[h] = R(0);
       if (in_c[h]) {
-        load_p(mats, a.B, C, k, ch[h], c, P);
-        Uc[h] = mv_t(P, Ut[h]);
-        m[h] = max4(Uc[h]);
-      }
-    }
-    if (in_c[0] || in_c[1]) {
-      xsum(m, 2, true);
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        if (in_c[h]) {
-          scale4(Uc[h], scale_exp(m[h]));
-          if constexpr (LV == 1) {
-            if (val) st4(a.partials + (prow(a.T, a.K, C, a.np, ch[h], k, c) + n) * 4, Uc[h]);
-          }
-        }
-      }
-    }
-    if constexpr (LV == 2) {
-      if (in_c[0] || in_c[1]) {
-        V4<R> PLg[2][2], Utg[2][2];
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          d[h] = R(0);
-          if (in_c[h]) {
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-              resolve(gk[h][j], cur.Lg[h][j], cur.cg[h][j]);
-              load_p(mats, a.B, C, k, gk[h][j], c, P);
-              PLg[h][j] = mv(P, cur.Lg[h][j]);
-            }
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-              Utg[h][0].x[s] = Uc[h].x[s] * PLg[h][1].x[s];
-              Utg[h][1].x[s] = Uc[h].x[s] * PLg[h][0].x[s];
-              d[h] = fma(Utg[h][0].x[s], PLg[h][0].x[s], d[h]);
-            }
-            d[h] *= pc;
-          }
-        }
-        xsum(d, 2, false);
-        bool any_g = false;
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          if (in_c[h]) {
-            const R ch_coef = val ? w / d[h] * pc : R(0);
-            acc_outer(gg[h][0], ch_coef, Utg[h][0], cur.Lg[h][0]);
-            acc_outer(gg[h][1], ch_coef, Utg[h][1], cur.Lg[h][1]);
-          }
-#pragma unroll
-          for (int j = 0; j < 2; ++j) {
-            m[2 * h + j] = R(0);
-            if (in_g[h][j]) {
-              any_g = true;
-              load_p(mats, a.B, C, k, gk[h][j], c, P);
-              Utg[h][j] = mv_t(P, Utg[h][j]);
-              m[2 * h + j] = max4(Utg[h][j]);
-            }
-          }
-        }
-        if (any_g) {
-          xsum(m, 4, true);
-#pragma unroll
-          for (int h = 0; h < 2; ++h)
-#pragma unroll
-            for (int j = 0; j < 2; ++j)
-              if (in_g[h][j]) {
-                scale4(Utg[h][j], scale_exp(m[2 * h + j]));
-                if (val)
-                  st4(a.partials + (prow(a.T, a.K, C, a.np, gk[h][j], k, c) + n) * 4, Utg[h][j]);
-              }
-        }
+        getP(h, P);
+        V4<R> Uc = mv_t(P, Ut[h]);
+        scale4(Uc, scale_exp(qmax<CP>(max4(Uc))));
+        if (cval) st4(a.partials + (prow(a.T, a.K, C, a.np, ch[h], k, cc) + n) * 4, Uc);
       }
     }
   }
 
-  // Per-wave reduction buffers; when fewer than C fit, waves take turns.
-  R* buf = wbuf + (size_t)(c % wslots) * 1024;
-  const int rounds = (C + wslots - 1) / wslots;
-  for (int rd = 0; rd < rounds; ++rd) {
-    if (c / wslots == rd) {
+  // Chunk reduction of each branch statistic over the tile's 64 patterns:
+  // transpose through LDS, thread (ci = (c, entry), part) sums 16 patterns,
+  // then the 4 parts combine by shuffles (in double).
+  auto flush = [&](const R(&gv)[16], int b) {
+    const int part = t & 3, ci = t >> 2, c2 = ci >> 4, e = ci & 15;
+    double sum = 0.0;
 #pragma unroll
-      for (int h = 0; h < 2; ++h) flush_branch(a, mats_dt, g[h], buf, ch[h], k, c, chunk);
-      if constexpr (LV == 2) {
+    for (int e0 = 0; e0 < 16; e0 += NE) {
+      __syncthreads();
 #pragma unroll
-        for (int h = 0; h < 2; ++h)
-          if (in_c[h])
+      for (int i = 0; i < NE; ++i) gb[(pl * CP + c) * (NE + 1) + i] = gv[e0 + i];
+      __syncthreads();
+      if (e >= e0 && e < e0 + NE) {
 #pragma unroll
-            for (int j = 0; j < 2; ++j)
-              flush_branch(a, mats_dt, gg[h][j], buf, gk[h][j], k, c, chunk);
+        for (int j = 0; j < 16; ++j)
+          sum += (double)gb[((part * 16 + j) * CP + c2) * (NE + 1) + (e - e0)];
       }
     }
-    if (rounds > 1) lds_barrier();
-  }
+    sum += __shfl_xor(sum, 1);
+    sum += __shfl_xor(sum, 2);
+    if (mats_dt != nullptr) {
+      // lanes (c2, e, part) of one category c2 form one wave: contract over e
+      double tv = (part == 0 && c2 < C)
+                      ? sum * (double)mats_dt[(((size_t)k * a.B + b) * C + c2) * 16 + e] : 0.0;
+#pragma unroll
+      for (int o = 4; o < 64; o <<= 1) tv += __shfl_xor(tv, o);
+      if ((t & 63) == 0 && c2 < C) a.gtau[(((size_t)b * a.K + k) * a.nchunks + chunk) * C + c2] = tv;
+    } else if (part == 0 && c2 < C) {
+      a.gpart[((((size_t)b * a.K + k) * a.nchunks + chunk) * C + c2) * 16 + e] = (R)sum;
+    }
+  };
+#pragma unroll
+  for (int h = 0; h < 2; ++h) flush(g[h], ch[h]);
 }
 
 // ------------------------------------------------------------------ launchers
@@ -792,44 +620,30 @@ int launch_post_dna(const PassArgs<R>& a, const int32_t* prog, const int32_t* se
 }
 
 template <typename R>
-int dna_pre_levels(int C) {
-  return (std::is_same<R, float>::value && C <= 4) ? 2 : 1;
-}
-
-template <typename R>
 int launch_pre_dna(const PassArgs<R>& a, const int32_t* nodes, int width, int tips, bool root,
                    hipStream_t st) {
-  const size_t nt = (size_t)a.C * 64, rb = sizeof(R);
-  const int lv = dna_pre_levels<R>(a.C);
-  const size_t nx = (size_t)next_pow2(a.C) * 64;
-  const size_t fixed = 2 * (lv == 2 ? 4 : 2) * nx * rb + tbl_bytes(tips, a.ncodes, rb);
-  int wslots = a.C;
-  while (wslots > 1 && fixed + (size_t)wslots * 1024 * rb > kDnaLdsBudget) --wslots;
-  const size_t lds = fixed + (size_t)wslots * 1024 * rb;
+  const int cp = next_pow2(a.C);
+  const size_t rb = sizeof(R);
+  const int ne = pre_pass_entries(cp, (int)rb);
+  const size_t lds = (size_t)2 * a.C * 16 * rb + tbl_bytes(tips, a.ncodes, rb) +
+                     (size_t)64 * cp * (ne + 1) * rb;
   HX_REQUIRE(lds <= kDnaLdsBudget, HX_ERR_UNSUPPORTED, "pre-order LDS exceeds the 64 KiB budget");
   dim3 grid(a.nchunks, a.K, width);
-  const int threads = a.C * 64;
-#define HX_PRE(TV, RV, LVV, CMV) \
-  k_pre_grp<R, TV, RV, LVV, CMV><<<grid, threads, lds, st>>>(a, a.mats, a.mats_dt, nodes, wslots)
-#define HX_PRE_T(LVV, CMV)                                                               \
-  if (tips == HX_TIPS_CODES) {                                                           \
-    if (root) HX_PRE(HX_TIPS_CODES, true, LVV, CMV); else HX_PRE(HX_TIPS_CODES, false, LVV, CMV); \
-  } else {                                                                               \
-    if (root) HX_PRE(HX_TIPS_DENSE, true, LVV, CMV); else HX_PRE(HX_TIPS_DENSE, false, LVV, CMV); \
+  const int threads = 64 * cp;
+#define HX_PRE(TV, RV, CPV) \
+  k_pre<R, TV, RV, CPV><<<grid, threads, lds, st>>>(a, a.mats, a.mats_dt, nodes)
+#define HX_PRE_T(CPV)                                                                        \
+  if (tips == HX_TIPS_CODES) {                                                               \
+    if (root) HX_PRE(HX_TIPS_CODES, true, CPV); else HX_PRE(HX_TIPS_CODES, false, CPV);      \
+  } else {                                                                                   \
+    if (root) HX_PRE(HX_TIPS_DENSE, true, CPV); else HX_PRE(HX_TIPS_DENSE, false, CPV);      \
   }
-  const int cm = next_pow2(a.C);
-  if (lv == 2) {
-    if constexpr (std::is_same<R, float>::value) {
-      if (cm == 1) { HX_PRE_T(2, 1) } else if (cm == 2) { HX_PRE_T(2, 2) } else { HX_PRE_T(2, 4) }
-    }
-  } else {
-    switch (cm) {
-      case 1: HX_PRE_T(1, 1) break;
-      case 2: HX_PRE_T(1, 2) break;
-      case 4: HX_PRE_T(1, 4) break;
-      case 8: HX_PRE_T(1, 8) break;
-      default: HX_PRE_T(1, 16) break;
-    }
+  switch (cp) {
+    case 1: HX_PRE_T(1) break;
+    case 2: HX_PRE_T(2) break;
+    case 4: HX_PRE_T(4) break;
+    case 8: HX_PRE_T(8) break;
+    default: HX_PRE_T(16) break;
   }
 #undef HX_PRE_T
 #undef HX_PRE
@@ -845,7 +659,5 @@ template int launch_pre_dna<float>(const PassArgs<float>&, const int32_t*, int, 
                                    hipStream_t);
 template int launch_pre_dna<double>(const PassArgs<double>&, const int32_t*, int, int, bool,
                                     hipStream_t);
-template int dna_pre_levels<float>(int);
-template int dna_pre_levels<double>(int);
 
 }  // namespace hx
