@@ -122,9 +122,8 @@ static WsLayout ws_layout(const hx_plan_s* plan, int S, int N, int C, int K, int
 template <typename R>
 static int launch_dense(bool post, const PassArgs<R>& a, const int32_t* nodes, int width, int tips,
                         bool root, int S, hipStream_t st) {
-  if constexpr (std::is_same<R, float>::value) {
-    if (use_mfma(a.C, S, HX_F32)) return launch_mfma(post, a, nodes, width, tips, root, S, st);
-  }
+  if (use_mfma(a.C, S, std::is_same<R, float>::value ? HX_F32 : HX_F64))
+    return launch_mfma<R>(post, a, nodes, width, tips, root, S, st);
   return post ? launch_post_wide<R>(a, nodes, width, tips, root, S, st)
               : launch_pre_wide<R>(a, nodes, width, tips, root, S, st);
 }

@@ -53,9 +53,11 @@ template <typename R>
 int launch_pre_dna(const PassArgs<R>& a, const int32_t* nodes, int width, int tips, bool root,
                    hipStream_t st);
 
-// FP32 dense-state kernels on MFMA (pruning_mfma.hip); used when use_mfma().
+// Dense-state kernels on MFMA (pruning_mfma.hip, FP32 and FP64); used when
+// use_mfma().
 bool use_mfma(int C, int S, int dtype);
-int launch_mfma(bool post, const PassArgs<float>& a, const int32_t* nodes, int width, int tips,
+template <typename R>
+int launch_mfma(bool post, const PassArgs<R>& a, const int32_t* nodes, int width, int tips,
                 bool root, int S, hipStream_t st);
 
 }  // namespace hx

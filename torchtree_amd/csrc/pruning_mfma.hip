@@ -1,4 +1,5 @@
-// FP32 pruning kernels for dense state spaces on the matrix cores (MFMA).
+// Pruning kernels for dense state spaces on the matrix cores (MFMA), FP32
+// (v_mfma_f32_16x16x4f32) and FP64 (v_mfma_f64_16x16x4f64).
 //
 // Reference: calculate_treelikelihood_discrete* (torchtree/evolution/
 // tree_likelihood.py:40-278) at S = 20 (amino acids) and 61 (codons), where
@@ -17,8 +18,8 @@
 // so each lane's A operand is a contiguous run and the accumulator layout of
 // one product is the operand layout of the next. The 16x16x4 f32 layout used:
 // A[i][k] from lane i + 16 k, B[k][j] from lane j + 16 k, D[i][j] in lane
-// j + 16 (i / 4), register i % 4 (checked on the device by
-// hx_selftest_mfma_layout).
+// j + 16 (i / 4), register i % 4 for FP32 and lane j + 16 (i % 4), register
+// i / 4 for FP64 (acc_row; checked on the device by hx_selftest_mfma_layout).
 #include "pruning.hpp"
 
 #include <math.h>
@@ -27,16 +28,39 @@ namespace hx {
 
 namespace {
 
-typedef float f4 __attribute__((ext_vector_type(4)));
+template <typename R>
+using vec4_t = R __attribute__((ext_vector_type(4)));
 constexpr int kTile = 16;
 
-__device__ __forceinline__ f4 mma(float a, float b, f4 c) {
+// One 16x16x4 product D += A B on the matrix cores; FP32 and FP64 use the
+// same operand and accumulator lane layout (checked on the device by
+// hx_selftest_mfma_layout).
+__device__ __forceinline__ vec4_t<float> mma(float a, float b, vec4_t<float> c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
+__device__ __forceinline__ vec4_t<double> mma(double a, double b, vec4_t<double> c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
 
-__device__ __forceinline__ float row_max16(float v) {
+__device__ __forceinline__ float fmax_(float a, float b) { return fmaxf(a, b); }
+__device__ __forceinline__ double fmax_(double a, double b) { return fmax(a, b); }
+__device__ __forceinline__ float frexp_(float m, int* e) { return frexpf(m, e); }
+__device__ __forceinline__ double frexp_(double m, int* e) { return frexp(m, e); }
+__device__ __forceinline__ float ldexp_(float x, int e) { return ldexpf(x, e); }
+__device__ __forceinline__ double ldexp_(double x, int e) { return ldexp(x, e); }
+
+// Row of the 16x16 accumulator held by lane group g (lane = column + 16 g)
+// in register r: FP32 keeps rows 4g..4g+3 in a lane, FP64 rows g, g+4, g+8,
+// g+12 (checked on the device by hx_selftest_mfma_layout).
+template <typename R>
+__device__ __forceinline__ constexpr int acc_row(int g, int r) {
+  return sizeof(R) == 4 ? 4 * g + r : g + 4 * r;
+}
+
+template <typename R>
+__device__ __forceinline__ R row_max16(R v) {
 #pragma unroll
-  for (int o = 1; o < 16; o <<= 1) v = fmaxf(v, __shfl_xor(v, o));
+  for (int o = 1; o < 16; o <<= 1) v = fmax_(v, __shfl_xor(v, o));
   return v;
 }
 
@@ -48,10 +72,10 @@ __device__ __forceinline__ T row_sum16(T v) {
 }
 
 // Child partial of child ch at absolute pattern n, category c, state s (0 outside).
-template <int TIPS>
-__device__ __forceinline__ float child_val(const PassArgs<float>& a, int ch, int k, int n, int c,
-                                           int s, int S) {
-  if (n >= a.N || s >= S) return 0.f;
+template <typename R, int TIPS>
+__device__ __forceinline__ R child_val(const PassArgs<R>& a, int ch, int k, int n, int c, int s,
+                                       int S) {
+  if (n >= a.N || s >= S) return R(0);
   if (ch < a.T) {
     if constexpr (TIPS == HX_TIPS_CODES) {
       return a.table[(size_t)a.codes[(size_t)ch * a.N + n] * S + s];
@@ -65,14 +89,14 @@ __device__ __forceinline__ float child_val(const PassArgs<float>& a, int ch, int
 }  // namespace
 
 // ---------------------------------------------------------------- post-order
-template <int MT, int TIPS, bool ROOT>
-__global__ __launch_bounds__(512) void k_post_mfma(PassArgs<float> a,
+template <typename R, int MT, int TIPS, bool ROOT>
+__global__ __launch_bounds__(512) void k_post_mfma(PassArgs<R> a,
                                                    const int32_t* __restrict__ nodes, int S) {
   constexpr int SP = 16 * MT, KS = 4 * MT;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int C = a.C;
-  float* Yx = reinterpret_cast<float*>(smem_raw);       // [C][16][SP] child-1 products
-  float* mx = Yx + C * kTile * SP;                      // [C][16]
+  R* Yx = reinterpret_cast<R*>(smem_raw);       // [C][16][SP] child-1 products
+  R* mx = Yx + C * kTile * SP;                      // [C][16]
   double* pxs = reinterpret_cast<double*>(mx + C * kTile);  // [C][16]
   double* red = pxs + C * kTile;                        // [C][SP] root reductions
   int* Et = reinterpret_cast<int*>(red + C * SP);       // [2][16]
@@ -85,25 +109,25 @@ __global__ __launch_bounds__(512) void k_post_mfma(PassArgs<float> a,
   const int ch0 = a.kids[2 * v], ch1 = a.kids[2 * v + 1];
   const int chh = h ? ch1 : ch0;
 
-  float Pf[MT][KS];  // Pf[t][ks] = P[16 t + nn][g KS + ks]
+  R Pf[MT][KS];  // Pf[t][ks] = P[16 t + nn][g KS + ks]
   {
-    const float* pm = a.mats + (((size_t)k * a.B + chh) * C + c) * S * S;
+    const R* pm = a.mats + (((size_t)k * a.B + chh) * C + c) * S * S;
 #pragma unroll
     for (int t = 0; t < MT; ++t)
 #pragma unroll
       for (int ks = 0; ks < KS; ++ks) {
         const int i = 16 * t + nn, j = g * KS + ks;
-        Pf[t][ks] = (i < S && j < S) ? pm[i * S + j] : 0.f;
+        Pf[t][ks] = (i < S && j < S) ? pm[i * S + j] : R(0);
       }
   }
-  float pi_t[MT];
-  float pc = 0.f;
+  R pi_t[MT];
+  R pc = R(0);
   double acc_l = 0.0, acc_p = 0.0, acc_f[MT];
   if constexpr (ROOT) {
 #pragma unroll
     for (int t = 0; t < MT; ++t) {
       const int i = 16 * t + nn;
-      pi_t[t] = i < S ? a.freqs[(size_t)k * a.freqs_ks + i] : 0.f;
+      pi_t[t] = i < S ? a.freqs[(size_t)k * a.freqs_ks + i] : R(0);
       acc_f[t] = 0.0;
     }
     pc = a.props[(size_t)k * a.props_ks + c];
@@ -119,12 +143,12 @@ __global__ __launch_bounds__(512) void k_post_mfma(PassArgs<float> a,
       Et[threadIdx.x] =
           (n < a.N && chq >= a.T) ? a.scales[((size_t)(chq - a.T) * a.K + k) * a.N + n] : 0;
     }
-    f4 acc[MT];
+    vec4_t<R> acc[MT];
 #pragma unroll
-    for (int t = 0; t < MT; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int t = 0; t < MT; ++t) acc[t] = vec4_t<R>{R(0), R(0), R(0), R(0)};
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
-      const float av = child_val<TIPS>(a, chh, k, nbase + nn, c, g * KS + ks, S);
+      const R av = child_val<R, TIPS>(a, chh, k, nbase + nn, c, g * KS + ks, S);
 #pragma unroll
       for (int t = 0; t < MT; ++t) acc[t] = mma(av, Pf[t][ks], acc[t]);
     }
@@ -133,35 +157,35 @@ __global__ __launch_bounds__(512) void k_post_mfma(PassArgs<float> a,
 #pragma unroll
       for (int t = 0; t < MT; ++t)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) Yx[(c * kTile + 4 * g + r) * SP + 16 * t + nn] = acc[t][r];
+        for (int r = 0; r < 4; ++r) Yx[(c * kTile + acc_row<R>(g, r)) * SP + 16 * t + nn] = acc[t][r];
     }
     __syncthreads();
-    float x[MT][4];
+    R x[MT][4];
     if (h == 0) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        float m_ = 0.f;
+        R m_ = R(0);
 #pragma unroll
         for (int t = 0; t < MT; ++t) {
-          x[t][r] = acc[t][r] * Yx[(c * kTile + 4 * g + r) * SP + 16 * t + nn];
-          m_ = fmaxf(m_, x[t][r]);
+          x[t][r] = acc[t][r] * Yx[(c * kTile + acc_row<R>(g, r)) * SP + 16 * t + nn];
+          m_ = fmax_(m_, x[t][r]);
         }
         m_ = row_max16(m_);
-        if (nn == 0) mx[c * kTile + 4 * g + r] = m_;
+        if (nn == 0) mx[c * kTile + acc_row<R>(g, r)] = m_;
       }
     }
     __syncthreads();
     if (h == 0) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int m = 4 * g + r, n = nbase + m;
-        float M = 0.f;
-        for (int cc = 0; cc < C; ++cc) M = fmaxf(M, mx[cc * kTile + m]);
+        const int m = acc_row<R>(g, r), n = nbase + m;
+        R M = R(0);
+        for (int cc = 0; cc < C; ++cc) M = fmax_(M, mx[cc * kTile + m]);
         int e = 0;
-        if (M > 0.f && isfinite(M)) {
-          frexpf(M, &e);
+        if (M > R(0) && isfinite(M)) {
+          frexp_(M, &e);
 #pragma unroll
-          for (int t = 0; t < MT; ++t) x[t][r] = ldexpf(x[t][r], -e);
+          for (int t = 0; t < MT; ++t) x[t][r] = ldexp_(x[t][r], -e);
         }
         const int E = Et[m] + Et[kTile + m] + e;
         if constexpr (!ROOT) {
@@ -173,9 +197,9 @@ __global__ __launch_bounds__(512) void k_post_mfma(PassArgs<float> a,
             if (c == 0 && nn == 0) a.scales[row] = E;
           }
         } else {
-          float p_ = 0.f;
+          R p_ = R(0);
 #pragma unroll
-          for (int t = 0; t < MT; ++t) p_ = fmaf(pi_t[t], x[t][r], p_);
+          for (int t = 0; t < MT; ++t) p_ = fma(pi_t[t], x[t][r], p_);
           p_ = row_sum16(p_);
           if (nn == 0) pxs[c * kTile + m] = (double)p_;
           if (c == 0 && nn == 0) Er[m] = E;
@@ -187,11 +211,11 @@ __global__ __launch_bounds__(512) void k_post_mfma(PassArgs<float> a,
       if (h == 0) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int m = 4 * g + r, n = nbase + m;
+          const int m = acc_row<R>(g, r), n = nbase + m;
           if (n >= a.N) continue;
           double lik = 0.0;
           for (int cc = 0; cc < C; ++cc)
-            lik += (double)(a.props[(size_t)k * a.props_ks + cc] * (float)pxs[cc * kTile + m]);
+            lik += (double)(a.props[(size_t)k * a.props_ks + cc] * (R)pxs[cc * kTile + m]);
           const double w = a.weights[n];
           const double wi = w / lik;
           if (c == 0 && nn == 0) acc_l += w * (log(lik) + (double)Er[m] * kLn2);
@@ -232,15 +256,15 @@ __global__ __launch_bounds__(512) void k_post_mfma(PassArgs<float> a,
 }
 
 // ----------------------------------------------------------------- pre-order
-template <int MT, int TIPS, bool ROOT>
-__global__ __launch_bounds__(512) void k_pre_mfma(PassArgs<float> a,
+template <typename R, int MT, int TIPS, bool ROOT>
+__global__ __launch_bounds__(512) void k_pre_mfma(PassArgs<R> a,
                                                   const int32_t* __restrict__ nodes, int S) {
   constexpr int SP = 16 * MT, KS = 4 * MT;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int C = a.C;
-  float* Yx = reinterpret_cast<float*>(smem_raw);  // [2][C][16][SP]: P L, then Ut transposes
-  float* dens = Yx + 2 * C * kTile * SP;           // [C][16]
-  float* mxx = dens + C * kTile;                   // [2][C][16]
+  R* Yx = reinterpret_cast<R*>(smem_raw);  // [2][C][16][SP]: P L, then Ut transposes
+  R* dens = Yx + 2 * C * kTile * SP;           // [C][16]
+  R* mxx = dens + C * kTile;                   // [2][C][16]
 
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int h = wave / C, c = wave - h * C;
@@ -249,49 +273,49 @@ __global__ __launch_bounds__(512) void k_pre_mfma(PassArgs<float> a,
   const int ch0 = a.kids[2 * p], ch1 = a.kids[2 * p + 1];
   const int chh = h ? ch1 : ch0;
   const bool internal = chh >= a.T;
-  const float* pm = a.mats + (((size_t)k * a.B + chh) * C + c) * S * S;
+  const R* pm = a.mats + (((size_t)k * a.B + chh) * C + c) * S * S;
 
   // Pf[t][ks] = P[16 t + nn][g KS + ks]; at 64 padded states the gradient
   // accumulators take those registers and P is re-read from L1/L2 per tile
   constexpr bool kPfRegs = MT < 4;
-  float Pf[kPfRegs ? MT : 1][kPfRegs ? KS : 1];
+  R Pf[kPfRegs ? MT : 1][kPfRegs ? KS : 1];
   if constexpr (kPfRegs) {
 #pragma unroll
     for (int t = 0; t < MT; ++t)
 #pragma unroll
       for (int ks = 0; ks < KS; ++ks) {
         const int i = 16 * t + nn, j = g * KS + ks;
-        Pf[t][ks] = (i < S && j < S) ? pm[i * S + j] : 0.f;
+        Pf[t][ks] = (i < S && j < S) ? pm[i * S + j] : R(0);
       }
   }
-  const float pc = a.props[(size_t)k * a.props_ks + c];
-  f4 G[MT][MT];  // G[Mt][Nt] lane (nn, g) reg r = dlogL/dP[16 Mt + 4 g + r][16 Nt + nn]
+  const R pc = a.props[(size_t)k * a.props_ks + c];
+  vec4_t<R> G[MT][MT];  // G[Mt][Nt] lane (nn, g) reg r = dlogL/dP[16 Mt + 4 g + r][16 Nt + nn]
 #pragma unroll
   for (int i = 0; i < MT; ++i)
 #pragma unroll
-    for (int j = 0; j < MT; ++j) G[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-  float* mine = Yx + (size_t)(h * C + c) * kTile * SP;
-  const float* other = Yx + (size_t)((1 - h) * C + c) * kTile * SP;
+    for (int j = 0; j < MT; ++j) G[i][j] = vec4_t<R>{R(0), R(0), R(0), R(0)};
+  R* mine = Yx + (size_t)(h * C + c) * kTile * SP;
+  const R* other = Yx + (size_t)((1 - h) * C + c) * kTile * SP;
 
   const int t0 = chunk * a.tiles_per_chunk;
   const int t1 = min(t0 + a.tiles_per_chunk, a.ntiles);
   for (int tile = t0; tile < t1; ++tile) {
     const int nbase = tile * kTile;
     // forward P_h L_h
-    f4 acc[MT];
+    vec4_t<R> acc[MT];
 #pragma unroll
-    for (int t = 0; t < MT; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int t = 0; t < MT; ++t) acc[t] = vec4_t<R>{R(0), R(0), R(0), R(0)};
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
-      const float av = child_val<TIPS>(a, chh, k, nbase + nn, c, g * KS + ks, S);
+      const R av = child_val<R, TIPS>(a, chh, k, nbase + nn, c, g * KS + ks, S);
 #pragma unroll
       for (int t = 0; t < MT; ++t) {
-        float bv;
+        R bv;
         if constexpr (kPfRegs) {
           bv = Pf[t][ks];
         } else {
           const int i = 16 * t + nn, j = g * KS + ks;
-          bv = (i < S && j < S) ? pm[i * S + j] : 0.f;
+          bv = (i < S && j < S) ? pm[i * S + j] : R(0);
         }
         acc[t] = mma(av, bv, acc[t]);
       }
@@ -299,18 +323,18 @@ __global__ __launch_bounds__(512) void k_pre_mfma(PassArgs<float> a,
 #pragma unroll
     for (int t = 0; t < MT; ++t)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) mine[(4 * g + r) * SP + 16 * t + nn] = acc[t][r];
+      for (int r = 0; r < 4; ++r) mine[(acc_row<R>(g, r)) * SP + 16 * t + nn] = acc[t][r];
     __syncthreads();
     // Ut_h = U_p * (P_other L_other), in the accumulator layout
-    float u[MT][4];
+    R u[MT][4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int m = 4 * g + r, n = nbase + m;
-      float d = 0.f;
+      const int m = acc_row<R>(g, r), n = nbase + m;
+      R d = R(0);
 #pragma unroll
       for (int t = 0; t < MT; ++t) {
         const int i = 16 * t + nn;
-        float up = 0.f;
+        R up = R(0);
         if (n < a.N && i < S) {
           if constexpr (ROOT) {
             up = a.freqs[(size_t)k * a.freqs_ks + i];
@@ -319,7 +343,7 @@ __global__ __launch_bounds__(512) void k_pre_mfma(PassArgs<float> a,
           }
         }
         u[t][r] = up * other[m * SP + i];
-        d = fmaf(u[t][r], acc[t][r], d);
+        d = fma(u[t][r], acc[t][r], d);
       }
       if (h == 0) {
         d = row_sum16(d);
@@ -328,25 +352,25 @@ __global__ __launch_bounds__(512) void k_pre_mfma(PassArgs<float> a,
     }
     __syncthreads();
     // gradient rows: G += (coef Ut)^T L over the tile's 16 patterns
-    float coef[4];
+    R coef[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int m = 4 * g + r, n = nbase + m;
-      float den = 0.f;
+      const int m = acc_row<R>(g, r), n = nbase + m;
+      R den = R(0);
       for (int cc = 0; cc < C; ++cc) den += dens[cc * kTile + m];
-      coef[r] = n < a.N ? (float)a.weights[n] / den * pc : 0.f;
+      coef[r] = n < a.N ? (R)a.weights[n] / den * pc : R(0);
     }
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
       // K index (ks, g) = pattern 4 g + ks: the A value of lane (nn, g) is its own
       // register u[Mt][ks] scaled by coef[ks]; B = L[4 g + ks][16 Nt + nn]
-      float bv[MT];
+      R bv[MT];
 #pragma unroll
       for (int nt = 0; nt < MT; ++nt)
-        bv[nt] = child_val<TIPS>(a, chh, k, nbase + 4 * g + ks, c, 16 * nt + nn, S);
+        bv[nt] = child_val<R, TIPS>(a, chh, k, nbase + acc_row<R>(g, ks), c, 16 * nt + nn, S);
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) {
-        const float av = coef[ks] * u[mt][ks];
+        const R av = coef[ks] * u[mt][ks];
 #pragma unroll
         for (int nt = 0; nt < MT; ++nt) G[mt][nt] = mma(av, bv[nt], G[mt][nt]);
       }
@@ -355,53 +379,53 @@ __global__ __launch_bounds__(512) void k_pre_mfma(PassArgs<float> a,
 #pragma unroll
     for (int t = 0; t < MT; ++t)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) mine[(4 * g + r) * SP + 16 * t + nn] = u[t][r];
+      for (int r = 0; r < 4; ++r) mine[(acc_row<R>(g, r)) * SP + 16 * t + nn] = u[t][r];
     __syncthreads();
-    f4 nu[MT];
+    vec4_t<R> nu[MT];
     if (internal) {
 #pragma unroll
-      for (int t = 0; t < MT; ++t) nu[t] = f4{0.f, 0.f, 0.f, 0.f};
+      for (int t = 0; t < MT; ++t) nu[t] = vec4_t<R>{R(0), R(0), R(0), R(0)};
 #pragma unroll
       for (int ks = 0; ks < KS; ++ks) {
         const int i = g * KS + ks;  // K index (ks, g) = state g KS + ks
-        const float av = mine[nn * SP + i];
+        const R av = mine[nn * SP + i];
 #pragma unroll
         for (int t = 0; t < MT; ++t) {
           const int j = 16 * t + nn;
-          const float bv = (i < S && j < S) ? pm[i * S + j] : 0.f;
+          const R bv = (i < S && j < S) ? pm[i * S + j] : R(0);
           nu[t] = mma(av, bv, nu[t]);
         }
       }
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        float m_ = 0.f;
+        R m_ = R(0);
 #pragma unroll
-        for (int t = 0; t < MT; ++t) m_ = fmaxf(m_, nu[t][r]);
+        for (int t = 0; t < MT; ++t) m_ = fmax_(m_, nu[t][r]);
         m_ = row_max16(m_);
-        if (nn == 0) mxx[(h * C + c) * kTile + 4 * g + r] = m_;
+        if (nn == 0) mxx[(h * C + c) * kTile + acc_row<R>(g, r)] = m_;
       }
     }
     __syncthreads();
     if (internal) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int m = 4 * g + r, n = nbase + m;
-        float M = 0.f;
-        for (int cc = 0; cc < C; ++cc) M = fmaxf(M, mxx[(h * C + cc) * kTile + m]);
+        const int m = acc_row<R>(g, r), n = nbase + m;
+        R M = R(0);
+        for (int cc = 0; cc < C; ++cc) M = fmax_(M, mxx[(h * C + cc) * kTile + m]);
         int e = 0;
-        if (M > 0.f && isfinite(M)) frexpf(M, &e);
+        if (M > R(0) && isfinite(M)) frexp_(M, &e);
         if (n < a.N) {
           const size_t row = ((size_t)(chh - a.T) * a.K + k) * a.N + n;
 #pragma unroll
           for (int t = 0; t < MT; ++t)
-            if (16 * t + nn < S) a.partials[(row * C + c) * S + 16 * t + nn] = ldexpf(nu[t][r], -e);
+            if (16 * t + nn < S) a.partials[(row * C + c) * S + 16 * t + nn] = ldexp_(nu[t][r], -e);
         }
       }
     }
     __syncthreads();
   }
   if (a.mats_dt != nullptr) {
-    const float* q = a.mats_dt + (((size_t)k * a.B + chh) * C + c) * S * S;
+    const R* q = a.mats_dt + (((size_t)k * a.B + chh) * C + c) * S * S;
     double s = 0.0;
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
@@ -409,7 +433,7 @@ __global__ __launch_bounds__(512) void k_pre_mfma(PassArgs<float> a,
       for (int nt = 0; nt < MT; ++nt)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int i = 16 * mt + 4 * g + r, j = 16 * nt + nn;
+          const int i = 16 * mt + acc_row<R>(g, r), j = 16 * nt + nn;
           if (i < S && j < S) s = fma((double)G[mt][nt][r], (double)q[i * S + j], s);
         }
 #pragma unroll
@@ -417,36 +441,47 @@ __global__ __launch_bounds__(512) void k_pre_mfma(PassArgs<float> a,
     if (lane == 0) a.gtau[(((size_t)chh * a.K + k) * a.nchunks + chunk) * C + c] = s;
   } else {
     const size_t css = (size_t)C * S * S;
-    float* o = a.gpart + (((size_t)chh * a.K + k) * a.nchunks + chunk) * css + (size_t)c * S * S;
+    R* o = a.gpart + (((size_t)chh * a.K + k) * a.nchunks + chunk) * css + (size_t)c * S * S;
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
       for (int nt = 0; nt < MT; ++nt)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int i = 16 * mt + 4 * g + r, j = 16 * nt + nn;
+          const int i = 16 * mt + acc_row<R>(g, r), j = 16 * nt + nn;
           if (i < S && j < S) o[i * S + j] = G[mt][nt][r];
         }
   }
 }
 
-bool use_mfma(int C, int S, int dtype) { return dtype == HX_F32 && S != 4 && C >= 1 && C <= 4; }
-
-static size_t mfma_lds(int C, int S, bool pre) {
-  const int MT = S <= 16 ? 1 : (S <= 32 ? 2 : 4);
-  const int SP = 16 * MT;
-  if (pre) return (size_t)(2 * C * kTile * SP + 3 * C * kTile) * 4 + 64;
-  return (size_t)(C * kTile * SP + C * kTile) * 4 + (size_t)(C * kTile + C * SP) * 8 + 3 * kTile * 4 +
-         64;
+bool use_mfma(int C, int S, int dtype) {
+  return (dtype == HX_F32 || dtype == HX_F64) && S != 4 && C >= 1 && C <= 4;
 }
 
-int launch_mfma(bool post, const PassArgs<float>& a, const int32_t* nodes, int width, int tips,
+template <typename R>
+static size_t mfma_lds(int C, int S, bool pre) {
+  const int MT = S <= 16 ? 1 : (S <= 32 ? 2 : 4);
+  const size_t SP = 16 * MT, rb = sizeof(R);
+  if (pre) return (2 * C * kTile * SP + 3 * C * kTile) * rb + 64;
+  return (C * kTile * SP + C * kTile) * rb + (size_t)(C * kTile + C * SP) * 8 + 3 * kTile * 4 + 64;
+}
+
+template <typename R>
+int launch_mfma(bool post, const PassArgs<R>& a, const int32_t* nodes, int width, int tips,
                 bool root, int S, hipStream_t st) {
   const int MT = S <= 16 ? 1 : (S <= 32 ? 2 : 4);
   const int threads = 2 * a.C * 64;
-  const size_t lds = mfma_lds(a.C, S, !post);
+  const size_t lds = mfma_lds<R>(a.C, S, !post);
+  HX_REQUIRE(lds <= 160 * 1024, HX_ERR_UNSUPPORTED, "dense-state kernel LDS exceeds 160 KiB");
   dim3 grid(a.nchunks, a.K, width);
-#define HX_MF(KER, MTV, TV, RV) KER<MTV, TV, RV><<<grid, threads, lds, st>>>(a, nodes, S)
+#define HX_MF(KER, MTV, TV, RV)                                                              \
+  do {                                                                                       \
+    auto kern = KER<R, MTV, TV, RV>;                                                         \
+    if (lds > 64 * 1024)                                                                     \
+      HX_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),                    \
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
+    kern<<<grid, threads, lds, st>>>(a, nodes, S);                                           \
+  } while (0)
 #define HX_MF_ALL(KER, MTV)                                                     \
   if (tips == HX_TIPS_CODES) {                                                  \
     if (root) HX_MF(KER, MTV, HX_TIPS_CODES, true); else HX_MF(KER, MTV, HX_TIPS_CODES, false); \
@@ -460,20 +495,49 @@ int launch_mfma(bool post, const PassArgs<float>& a, const int32_t* nodes, int w
   }
 #undef HX_MF_ALL
 #undef HX_MF
+  HX_HIP_TRY(hipGetLastError());
   return HX_OK;
 }
 
+template int launch_mfma<float>(bool, const PassArgs<float>&, const int32_t*, int, int, bool, int,
+                                hipStream_t);
+template int launch_mfma<double>(bool, const PassArgs<double>&, const int32_t*, int, int, bool, int,
+                                 hipStream_t);
+
 // ------------------------------------------------------- layout self-test
-// D = A B for one 16x16x4 f32 MFMA with A[i][k] = 1 + i + 16 k and
+// D = A B for one 16x16x4 MFMA with A[i][k] = 1 + i + 16 k and
 // B[k][j] = 1 + 0.5 j - k, read back through the assumed lane layouts.
-__global__ void k_mfma_layout(float* out) {
+template <typename R>
+__global__ void k_mfma_layout(R* out) {
   const int lane = threadIdx.x;
   const int i = lane & 15, kk = lane >> 4;
-  const float av = 1.f + i + 16.f * kk;         // A[i][k], lane = i + 16 k
-  const float bv = 1.f + 0.5f * i - (float)kk;  // B[k][j], lane = j + 16 k (j = lane & 15)
-  f4 d = mma(av, bv, f4{0.f, 0.f, 0.f, 0.f});
+  const R av = R(1) + R(i) + R(16) * R(kk);        // A[i][k], lane = i + 16 k
+  const R bv = R(1) + R(0.5) * R(i) - R(kk);       // B[k][j], lane = j + 16 k (j = lane & 15)
+  vec4_t<R> d = mma(av, bv, vec4_t<R>{R(0), R(0), R(0), R(0)});
 #pragma unroll
-  for (int r = 0; r < 4; ++r) out[(4 * (lane >> 4) + r) * 16 + (lane & 15)] = d[r];
+  for (int r = 0; r < 4; ++r) out[acc_row<R>(lane >> 4, r) * 16 + (lane & 15)] = d[r];
+}
+
+template <typename R>
+static int mfma_layout_check(hipStream_t st, int* bad) {
+  R* d = nullptr;
+  HX_HIP_TRY(hipMalloc(&d, 256 * sizeof(R)));
+  k_mfma_layout<R><<<1, 64, 0, st>>>(d);
+  R host[256];
+  hipError_t e = hipMemcpyAsync(host, d, sizeof(host), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  hipFree(d);
+  if (e != hipSuccess) {
+    set_error(std::string("mfma self-test: ") + hipGetErrorString(e));
+    return HX_ERR_HIP;
+  }
+  for (int i = 0; i < 16; ++i)
+    for (int j = 0; j < 16; ++j) {
+      double ref = 0.0;
+      for (int kk = 0; kk < 4; ++kk) ref += (1.0 + i + 16.0 * kk) * (1.0 + 0.5 * j - kk);
+      if (fabs((double)host[i * 16 + j] - ref) > 1e-3 * (1.0 + fabs(ref))) ++*bad;
+    }
+  return HX_OK;
 }
 
 }  // namespace hx
@@ -481,24 +545,9 @@ __global__ void k_mfma_layout(float* out) {
 extern "C" int hx_selftest_mfma_layout(int* mismatches, hx_stream_t stream) {
   HX_REQUIRE(mismatches != nullptr, HX_ERR_INVALID_ARGUMENT, "null argument");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  float* d = nullptr;
-  HX_HIP_TRY(hipMalloc(&d, 256 * sizeof(float)));
-  hx::k_mfma_layout<<<1, 64, 0, st>>>(d);
-  float host[256];
-  hipError_t e = hipMemcpyAsync(host, d, sizeof(host), hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess) e = hipStreamSynchronize(st);
-  hipFree(d);
-  if (e != hipSuccess) {
-    hx::set_error(std::string("mfma self-test: ") + hipGetErrorString(e));
-    return HX_ERR_HIP;
-  }
   int bad = 0;
-  for (int i = 0; i < 16; ++i)
-    for (int j = 0; j < 16; ++j) {
-      double ref = 0.0;
-      for (int kk = 0; kk < 4; ++kk) ref += (1.0 + i + 16.0 * kk) * (1.0 + 0.5 * j - kk);
-      if (fabs(host[i * 16 + j] - ref) > 1e-3 * (1.0 + fabs(ref))) ++bad;
-    }
+  int rc = hx::mfma_layout_check<float>(st, &bad);
+  if (rc == HX_OK) rc = hx::mfma_layout_check<double>(st, &bad);
   *mismatches = bad;
-  return HX_OK;
+  return rc;
 }

@@ -81,7 +81,7 @@ __device__ __forceinline__ void fill_children(const PassArgs<R>& a, R* Lt, int c
 }  // namespace
 
 int wide_nt(int C, int S, int rb) {
-  if (rb == 4 && use_mfma(C, S, HX_F32)) return 16;  // MFMA tiles of 16 patterns
+  if (use_mfma(C, S, rb == 8 ? HX_F64 : HX_F32)) return 16;  // MFMA tiles of 16 patterns
   const int SP = S <= 32 ? 32 : 64;
   const int G = C * SP;
   int nt = kWideLdsBytes / (7 * G * rb);
