@@ -54,9 +54,12 @@ def parse():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-mix measurement")
     ap.add_argument("--serve-jobs", type=int, default=200)
     ap.add_argument("--serve-requests", type=int, default=200)
-    ap.add_argument("--serve-plain-limit", type=float, default=60.0,
+    ap.add_argument("--serve-plain-limit", type=float, default=150.0,
                     help="seconds of sequential plain FP64 evaluation (stats over completed requests)")
     ap.add_argument("--serve-streams", type=int, default=4)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="first run one device-to-device copy of BYTES (a known-size kernel "
+                         "for scaling the FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
 
 
@@ -112,6 +115,14 @@ def main():
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.copy_calibration > 0:
+        src = torch.ones(args.copy_calibration, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {args.copy_calibration} bytes")
+        del src, dst
     t0 = time.time()
     wl = make_config(args, synthetic)
     log(f"workload {wl.name}: T={wl.T} N={wl.N} C={wl.C} K={wl.K} S={wl.S} built in {time.time()-t0:.1f}s")

@@ -225,19 +225,22 @@ __global__ __launch_bounds__(CM >= 8 ? 256 : 1024) void k_post_walk(PassArgs<R> 
     else return val ? ld4(a.dense + ((size_t)ch * a.N + n) * 4) : zero4<R>();
   };
   // Issued every node, for the stacked child of the next entry or, when it
-  // has none, for slot 0 (a valid address, value unused): every iteration
-  // then issues the same vector memory operations, so the wait for these
-  // loads counts only the older ones and not this node's stores.
+  // has none, from this sample's P blocks (a few hot cache lines, value
+  // unused): every iteration then issues the same vector memory operations,
+  // so the wait for these loads counts only the older ones and not this
+  // node's stores, and the placeholder loads never reach HBM.
   auto stacked_of = [&](const int4& f0, const int4& f1) {
     if ((f1.z & 0xff) == 2) return f0.y;
     if (((f1.z >> 8) & 0xff) == 2) return f0.z;
-    return a.T;
+    return -1;
   };
+  const R* hot = mats + (size_t)k * a.B * C * 16 + (lane & 3) * 4;
   auto readback = [&](int ch, V4<R>(&x)[CM]) {
-    const R* p = base + (size_t)(ch - a.T) * sstride;
+    const bool real = ch >= 0;
+    const R* p = base + (size_t)(real ? ch - a.T : 0) * sstride;
 #pragma unroll
     for (int c = 0; c < CM; ++c)
-      if (EXACT || c < C) x[c] = ld4(p + (size_t)c * cstride);
+      if (EXACT || c < C) x[c] = ld4(real ? p + (size_t)c * cstride : hot);
   };
 
   // The first entry of a walk has two tip children, so xs is first read
@@ -376,8 +379,7 @@ __global__ __launch_bounds__(CM >= 8 ? 256 : 1024) void k_post_walk(PassArgs<R> 
 // pattern in adjacent lanes: the likelihood sum and the rescaling maximum are
 // shuffles within those lanes, so the tile loop has no barriers. The two P
 // blocks are staged in LDS once per block; each branch statistic accumulates
-// in registers over the block's chunk of 64-pattern tiles, and every global
-// load of a tile is issued one tile ahead.
+// in registers over the block's chunk of 64-pattern tiles.
 template <typename R>
 struct PreTile {
   V4<R> U;
@@ -475,14 +477,14 @@ __global__ __launch_bounds__(64 * CP) void k_pre(PassArgs<R> a, const R* __restr
     }
   };
 
+  // No software prefetch of the next tile: its registers cost a wave per
+  // SIMD of occupancy, which hides the load latency better (measured).
   const int t0 = chunk * a.tiles_per_chunk;
   const int t1 = min(t0 + a.tiles_per_chunk, a.ntiles);
-  PreTile<R> nxt;
-  if (t0 < t1) fetch(t0, nxt);
   __syncthreads();  // P blocks and code table staged
   for (int tile = t0; tile < t1; ++tile) {
-    PreTile<R> cur = nxt;
-    if (tile + 1 < t1) fetch(tile + 1, nxt);
+    PreTile<R> cur;
+    fetch(tile, cur);
     const int n = tile * 64 + pl;
     const bool val = n < a.N && cval;
     const R w = val ? (R)a.weights[n] : R(0);
