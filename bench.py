@@ -223,7 +223,7 @@ def main():
             "kernel": (f"k_pre ({launches - 1} depth-level launches) + k_reduce_g"
                        if dom == "preorder" else "k_post_walk (one launch, whole tree) + k_reduce_root"),
             "achieved": dom_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dom_gbs / HBM_PEAK_GBS,
-            "traffic": None, "algorithmic_bytes": dom_b, "pass_ms": dom_ms,
+            "traffic": measured_traffic(args, dom), "algorithmic_bytes": dom_b, "pass_ms": dom_ms,
             "avg_launch_us": dom_ms * 1e3 / max(1, launches - 1),
         },
         "other_pass": {"postorder_gbs": post_gbs, "preorder_gbs": pre_gbs,
@@ -356,6 +356,20 @@ def serving_bench(args, dev):
         "capacity_vs_plain": cap["updates_per_s"] / pl["updates_per_s"],
         "max_logl_rel_err_vs_fp64": max(errs), "max_grad_rel_err_vs_fp64": max(gerrs),
     }
+
+
+def measured_traffic(args, dom):
+    """HBM bytes of the dominant pass per step, from the committed FETCH_SIZE /
+    WRITE_SIZE counter runs of this exact workload (rocprofv3, scaled by a
+    1 GiB copy; profiles/hbm_traffic_config2*.md), or None for other sizes."""
+    if (args.config, args.taxa, args.patterns, args.samples) != ("c2", 512, 100_000, 32):
+        return None
+    path = Path(__file__).resolve().parent / "profiles" / "hbm_traffic_config2.json"
+    try:
+        d = json.loads(path.read_text())[dom]
+    except (OSError, KeyError, ValueError):
+        return None
+    return d["read_bytes"] + d["write_bytes"]
 
 
 def _nchunks(N, C, K):
