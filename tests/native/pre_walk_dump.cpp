@@ -1,0 +1,29 @@
+// Reads T and (T-1) post-order triples on stdin, prints the pre-order walk
+// program built by hx::build_pre_walk: its stack depth, then one
+// "p a b usrc cont push tips" line per internal node. Host-only test driver for
+// the four-state pre-order walk.
+#include <cstdio>
+#include <vector>
+
+#include "walk.hpp"
+
+int main() {
+  int T = 0;
+  if (std::scanf("%d", &T) != 1 || T < 2) return 2;
+  const int I = T - 1, nn = 2 * T - 1;
+  std::vector<int32_t> post(3 * I), kids(2 * nn, -1);
+  for (int r = 0; r < I; ++r) {
+    if (std::scanf("%d %d %d", &post[3 * r], &post[3 * r + 1], &post[3 * r + 2]) != 3) return 2;
+    kids[2 * post[3 * r]] = post[3 * r + 1];
+    kids[2 * post[3 * r] + 1] = post[3 * r + 2];
+  }
+  std::vector<int32_t> prog;
+  int depth = 0;
+  if (!hx::build_pre_walk(T, kids, post.data(), prog, depth)) return 1;
+  std::printf("%d\n", depth);
+  for (int i = 0; i < I; ++i) {
+    const int32_t* e = prog.data() + hx::kPreInts * i;
+    std::printf("%d %d %d %d %d %d %d\n", e[0], e[1], e[2], e[3], e[4], e[5], e[6]);
+  }
+  return 0;
+}

@@ -138,3 +138,158 @@ def test_walk_balanced(walk_bin, T):
     if T & (T - 1) == 0:
         assert need == int(math.log2(T))  # perfectly balanced: log2 T entries
     _check(T, post, need, prog)
+
+
+# ---------------------------------------------------------------- pre-order walk
+@pytest.fixture(scope="module")
+def pre_walk_bin(tmp_path_factory):
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    out = tmp_path_factory.mktemp("prewalk") / "pre_walk_dump"
+    subprocess.run(["g++", "-O1", "-std=c++17", f"-I{CSRC}",
+                    str(HERE / "native" / "pre_walk_dump.cpp"), "-o", str(out)], check=True)
+    return out
+
+
+def _run_pre(pre_walk_bin, T, post):
+    inp = f"{T}\n" + "\n".join(f"{v} {a} {b}" for v, a, b in post) + "\n"
+    r = subprocess.run([str(pre_walk_bin)], input=inp, capture_output=True, text=True, check=True)
+    lines = [ln for ln in r.stdout.split("\n") if ln.strip()]
+    return int(lines[0]), [tuple(int(x) for x in ln.split()) for ln in lines[1:]]
+
+
+def _check_pre(T, post, depth, prog):
+    """Structural check: every internal node is a parent once, root first; a
+    continuation names a child of the previous entry; a pop finds its node on
+    top of the stack at the depth it was pushed; depth <= floor(log2 T)."""
+    kids = {v: (a, b) for v, a, b in post}
+    root = post[-1][0]
+    assert len(prog) == T - 1 and prog[0][0] == root and prog[0][3] == 0
+    seen, stack, nxt = set(), [], None
+    for p, a, b, usrc, cont, push, tips in prog:
+        assert kids[p] == (a, b) and p not in seen
+        seen.add(p)
+        if usrc == 1:
+            assert nxt == p
+        elif usrc != 0:
+            assert usrc & 0xFF == 2 and nxt is None
+            d = usrc >> 8
+            assert stack and stack[-1] == p and len(stack) - 1 == d
+            stack.pop()
+        assert tips == (a < T) | ((b < T) << 1)
+        internal = [h for h, x in enumerate((a, b)) if x >= T]
+        if len(internal) == 2:
+            assert push >= 0 and cont >= 0 and (push & 0xFF) == 1 - cont
+            assert push >> 8 == len(stack)
+            stack.append((a, b)[push & 0xFF])
+        else:
+            assert push == -1
+            assert cont == (internal[0] if internal else -1)
+        nxt = (a, b)[cont] if cont >= 0 else None
+        assert len(stack) <= depth
+    assert not stack and len(seen) == T - 1
+    assert depth <= max(0, int(math.floor(math.log2(T))))
+
+
+@pytest.mark.parametrize("T", [2, 3, 5, 17, 64, 512, 1024])
+def test_pre_walk_random(pre_walk_bin, T):
+    rng = np.random.default_rng(1000 + T)
+    topo = synthetic.random_topology(T, rng)
+    post = [tuple(int(x) for x in r) for r in topo.postorder]
+    depth, prog = _run_pre(pre_walk_bin, T, post)
+    _check_pre(T, post, depth, prog)
+
+
+@pytest.mark.parametrize("shape,T", [("caterpillar", 2), ("caterpillar", 50), ("balanced", 64),
+                                     ("balanced", 1000)])
+def test_pre_walk_shapes(pre_walk_bin, shape, T):
+    post = _caterpillar(T) if shape == "caterpillar" else _balanced(T)
+    depth, prog = _run_pre(pre_walk_bin, T, post)
+    _check_pre(T, post, depth, prog)
+    if shape == "caterpillar":
+        assert depth == 0
+    elif T & (T - 1) == 0:
+        assert depth == int(math.log2(T)) - 1
+
+
+def _pre_walk_statistics(T, post, prog, tips, mats, freqs, props, weights, rs):
+    """The kernel's algorithm restated in FP64 numpy: post-order partials are
+    overwritten in place by adjoints pushed at depth >= rs, shallower pushes
+    live in `rs` register slots; returns d loglik / d mats [B, C, S, S]."""
+    C, S = mats.shape[1], mats.shape[2]
+    post_l = {}
+    for t in range(T):
+        post_l[t] = np.broadcast_to(tips[t], (C, S, tips.shape[-1]))
+    for v, a, b in post:
+        post_l[v] = np.einsum("cij,cjn->cin", mats[a], post_l[a]) * np.einsum(
+            "cij,cjn->cin", mats[b], post_l[b])
+    mem = {v: ("L", post_l[v]) for v in range(T, 2 * T - 1)}
+    regs = [None] * rs
+    grad = np.zeros_like(mats)
+    U = None
+    for p, a, b, usrc, cont, push, tipf in prog:
+        if usrc == 0:
+            U = np.broadcast_to(freqs[:, None], (C, S, tips.shape[-1]))
+        elif usrc != 1:
+            d = usrc >> 8
+            if d < rs:
+                U = regs[d]
+            else:
+                kind, U = mem[p]
+                assert kind == "U"  # the in-place slot holds the pushed adjoint
+        L = []
+        for x in (a, b):
+            if x < T:
+                L.append(post_l[x])
+            else:
+                kind, val = mem[x]
+                assert kind == "L"  # read before any push overwrote it
+                L.append(val)
+        m = [np.einsum("cij,cjn->cin", mats[x], l) for x, l in zip((a, b), L)]
+        lik = np.einsum("c,cin->n", props, U * m[0] * m[1])
+        Ut = [U * m[1], U * m[0]]
+        for h, x in enumerate((a, b)):
+            grad[x] = np.einsum("n,c,cin,cjn->cij", weights / lik, props, Ut[h], L[h])
+
+        def adjoint(h):
+            un = np.einsum("cij,cin->cjn", mats[(a, b)[h]], Ut[h])
+            e = np.frexp(un.max(axis=(0, 1)))[1]
+            return np.ldexp(un, -e)
+
+        if push >= 0:
+            h, d = push & 0xFF, push >> 8
+            if d < rs:
+                regs[d] = adjoint(h)
+            else:
+                mem[(a, b)[h]] = ("U", adjoint(h))
+        U = adjoint(cont) if cont >= 0 else None
+    return grad
+
+
+@pytest.mark.parametrize("T,rs", [(2, 2), (9, 0), (40, 1), (40, 2), (130, 2)])
+def test_pre_walk_statistics_match_oracle(pre_walk_bin, T, rs):
+    """The walk's branch statistics equal the oracle's autograd gradient."""
+    import torch
+
+    from oracle import pruning as orc
+
+    rng = np.random.default_rng(7 * T + rs)
+    topo = synthetic.random_topology(T, rng)
+    post = [tuple(int(x) for x in r) for r in topo.postorder]
+    _, prog = _run_pre(pre_walk_bin, T, post)
+    C, S, N = 3, 4, 11
+    B = 2 * T - 2
+    mats = rng.dirichlet(np.ones(S), size=(B, C, S))  # rows sum to one
+    freqs = rng.dirichlet(np.ones(S))
+    props = rng.dirichlet(np.ones(C))
+    tips = rng.uniform(0.05, 1.0, size=(T, S, N))
+    weights = rng.integers(1, 5, size=N).astype(np.float64)
+    got = _pre_walk_statistics(T, post, prog, tips, mats, freqs, props, weights, rs)
+
+    tm = torch.tensor(mats, requires_grad=True)
+    parts = [torch.tensor(tips[t]) for t in range(T)] + [None] * (T - 1)
+    ll = orc.treelik_discrete(parts, torch.tensor(weights), post, tm, torch.tensor(freqs)[None],
+                              torch.tensor(props)[:, None, None])
+    ll.sum().backward()
+    want = tm.grad.numpy()
+    np.testing.assert_allclose(got, want, rtol=1e-11, atol=1e-12)

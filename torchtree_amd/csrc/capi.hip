@@ -101,6 +101,12 @@ int hx_plan_create(int taxa, const int32_t* postorder, int device, hx_plan_t* ou
     hx::set_error("walk program construction failed");
     return HX_ERR_BAD_TOPOLOGY;
   }
+  std::vector<int32_t> prew;
+  int prew_depth = 0;
+  if (!hx::build_pre_walk(T, kids, postorder, prew, prew_depth)) {
+    hx::set_error("pre-order walk program construction failed");
+    return HX_ERR_BAD_TOPOLOGY;
+  }
   auto* plan = new hx_plan_s();
   plan->device = device;
   plan->walk_need = walk_need;
@@ -110,9 +116,11 @@ int hx_plan_create(int taxa, const int32_t* postorder, int device, hx_plan_t* ou
   plan->kids = kids;
   const size_t walk_off = hx::align_up(2 * (size_t)nn + 2 * (size_t)I, 4);
   const size_t seg_off = walk_off + walk.size();
-  std::vector<int32_t> host(seg_off + segtab.size(), 0);
+  const size_t prew_off = hx::align_up(seg_off + segtab.size(), 4);
+  std::vector<int32_t> host(prew_off + prew.size(), 0);
   std::copy(walk.begin(), walk.end(), host.begin() + walk_off);
   std::copy(segtab.begin(), segtab.end(), host.begin() + seg_off);
+  std::copy(prew.begin(), prew.end(), host.begin() + prew_off);
   std::copy(kids.begin(), kids.end(), host.begin());
   int pos = 2 * nn;
   plan->post_off.push_back(0);
@@ -144,6 +152,8 @@ int hx_plan_create(int taxa, const int32_t* postorder, int device, hx_plan_t* ou
   plan->d_walk = plan->d_buf + walk_off;
   plan->d_segtab = plan->d_buf + seg_off;
   plan->nseg = (int)(segtab.size() / hx::kWalkSegTips);
+  plan->d_prew = plan->d_buf + prew_off;
+  plan->prew_depth = prew_depth;
   *out = plan;
   return HX_OK;
 }

@@ -65,6 +65,13 @@ int wide_nt(int C, int S, int rb);
 // sample's result is bitwise identical alone and in a batch.
 constexpr int kTargetChunks = 128;
 
+// The four-state FP32 pre-order walk with at most kPreWalkMaxC categories
+// sums its branch statistics over the kPreWalkW tiles of a workgroup: one
+// chunk per kPreWalkW tiles.
+constexpr int kPreWalkW = 4;
+constexpr int kPreWalkMaxC = 4;
+inline bool use_pre_walk(int S, int C, int rb) { return S == 4 && rb == 4 && C <= kPreWalkMaxC; }
+
 inline Tiling make_tiling(int N, int C, int K, int S, int rb) {
   (void)K;
   Tiling t;
@@ -77,6 +84,11 @@ inline Tiling make_tiling(int N, int C, int K, int S, int rb) {
     min_tiles = 4;
   }
   t.ntiles = (N + t.ppb - 1) / t.ppb;
+  if (use_pre_walk(S, C, rb)) {
+    t.tiles_per_chunk = kPreWalkW;
+    t.nchunks = (t.ntiles + kPreWalkW - 1) / kPreWalkW;
+    return t;
+  }
   int cap = t.ntiles / min_tiles;
   if (cap < 1) cap = 1;
   const int nch = cap < kTargetChunks ? cap : kTargetChunks;
@@ -107,4 +119,8 @@ struct hx_plan_s {
   const int32_t* d_segtab = nullptr;
   int nseg = 0;
   int walk_need = 0;
+  // Pre-order walk program (walk.hpp, build_pre_walk): kPreInts ints per
+  // internal node; its stack reaches `prew_depth` entries.
+  const int32_t* d_prew = nullptr;
+  int prew_depth = 0;
 };

@@ -52,6 +52,9 @@ int launch_post_dna(const PassArgs<R>& a, const int32_t* prog, const int32_t* se
 template <typename R>
 int launch_pre_dna(const PassArgs<R>& a, const int32_t* nodes, int width, int tips, bool root,
                    hipStream_t st);
+// The whole pre-order in one launch (FP32, C <= kPreWalkMaxC; use_pre_walk).
+int launch_pre_walk(const PassArgs<float>& a, const int32_t* prog, int nprog, int tips,
+                    hipStream_t st);
 
 // Dense-state kernels on MFMA (pruning_mfma.hip, FP32 and FP64); used when
 // use_mfma().

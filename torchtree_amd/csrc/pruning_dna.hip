@@ -557,6 +557,353 @@ __global__ __launch_bounds__(64 * CP) void k_pre(PassArgs<R> a, const R* __restr
   for (int h = 0; h < 2; ++h) flush(g[h], ch[h]);
 }
 
+// -------------------------------------------------------------- pre-order walk
+// Lane = pattern with all C categories in the lane, as in the post-order walk.
+// A block's W waves walk the same program (walk.hpp, build_pre_walk) for W
+// consecutive tiles of one sample, grid = (ceil(ntiles / W), K), so every
+// wave is at the same entry and the branch statistics of the W tiles are
+// summed in LDS once per entry. Entry (p; a, b) with p's adjoint U (the
+// frequencies at the root):
+//   m_h  = P_h L_h,  Ut_a = U * m_b,  Ut_b = U * m_a        (per category)
+//   lik  = sum_c p_c sum_s U m_a m_b                         (scaled site likelihood)
+//   G_h[c][i][j] = sum over patterns of (w p_c / lik) Ut_h[c][i] L_h[c][j]
+//   U_h  = P_h^T Ut_h with one power-of-two rescale over all categories
+// (tree_likelihood.py:40-278 through autograd). Every term of G is a ratio,
+// so the scales of U and L cancel. The pattern sum of G is a 16 x 64 x 16
+// product per branch on the matrix cores (v_mfma_f32_16x16x4f32): each wave
+// writes (w p_c / lik) Ut (rows 4c + i) and L (rows 4c + j) to LDS, pattern
+// minor, and reads them back as operands; the diagonal category blocks of the
+// product are G. Adjoints waiting on the walk's stack live in registers for
+// depths below RS and in the pushed child's own partials slot (in place, its
+// post-order partial has been read) deeper down.
+constexpr int kXS = 68;  // row stride (floats) of the LDS pattern transposes
+
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <int CM>
+struct PreIn {
+  V4<float> L[2][CM];
+  V4<float> Up[CM];
+  V4<float> tipv[2];  // dense tips: the tip's partial
+  int code[2];
+  float dt;  // dP/dtau entry this thread contracts with (d loglik / d tau mode)
+};
+
+template <int TIPS, int CM, bool EXACT, int RS, bool TAU>
+__global__ __launch_bounds__(256) void k_pre_walk(PassArgs<float> a, const float* __restrict__ mats,
+                                                  const float* __restrict__ mats_dt,
+                                                  const int* __restrict__ prog, int nprog, int W) {
+  using R = float;
+  using f4 = float __attribute__((ext_vector_type(4)));
+  using H0 = std::integral_constant<int, 0>;
+  using H1 = std::integral_constant<int, 1>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int C = EXACT ? CM : a.C;
+  const int wave = uni(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int chunk = blockIdx.x, k = blockIdx.y;
+  const int tile = chunk * W + wave;
+  // waves past the last tile repeat it with zero weight and store nothing
+  const bool tval = tile < a.ntiles;
+  const int tl = tval ? tile : a.ntiles - 1;
+  const int n = tl * 64 + lane;
+  const bool val = tval && n < a.N;
+  float* xt = reinterpret_cast<float*>(smem_raw) + (size_t)wave * 4 * 16 * kXS;  // [4][16][kXS]
+  float* red = reinterpret_cast<float*>(smem_raw) + (size_t)W * 4 * 16 * kXS;    // [2][2][W][64]
+  V4<R>* tbl = reinterpret_cast<V4<R>*>(red + 4 * W * 64);
+  for (int i = lane; i < 4 * 16 * kXS; i += 64) xt[i] = 0.f;  // rows of absent categories
+  stage_tbl<R, TIPS>(a, tbl, blockDim.x);
+  __syncthreads();
+
+  const R w = val ? (R)a.weights[n] : R(0);
+  R pc[CM];
+#pragma unroll
+  for (int c = 0; c < CM; ++c) pc[c] = (EXACT || c < C) ? a.props[(size_t)k * a.props_ks + c] : R(0);
+  V4<R> pi;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) pi.x[s] = a.freqs[(size_t)k * a.freqs_ks + s];
+
+  // Vector loads and stores address a wave-uniform base plus 16 bytes per
+  // lane: the (node, category) row of this tile, or for placeholders one
+  // cache line of this sample's P blocks that every wave keeps reading.
+  const size_t cstride = (size_t)a.np * 4;
+  auto row = [&](int node, int c) -> R* {
+    return a.partials + (((size_t)(node - a.T) * a.K + k) * C + c) * cstride + (size_t)tl * 256;
+  };
+  const R* hot = mats + (size_t)k * a.B * C * 16;
+  const int off_real = lane * 4, off_hot = (lane & 3) * 4;
+  const cptr<R> mk = to_const(mats) + (size_t)k * a.B * C * 16;
+  const cptr<int> cprog = to_const(prog);
+  auto entry = [&](int i, int4& f0, int4& f1) {
+    const cptr<int> q = cprog + (size_t)kPreInts * i;
+    f0.x = q[0], f0.y = q[1], f0.z = q[2], f0.w = q[3];
+    f1.x = q[4], f1.y = q[5], f1.z = q[6], f1.w = q[7];
+  };
+  // Every entry issues the same vector loads (placeholders where the entry
+  // has nothing to read), so the wait for one entry's data counts the next
+  // entry's loads exactly and never drains them.
+  auto fetch = [&](const int4& f0, const int4& f1, PreIn<CM>& in) {
+    if constexpr (TAU) {
+      // issued first: the copy of this value into the loop-carried register
+      // then waits for this load only
+      const int CS = C * 16, t2 = threadIdx.x % (2 * CS);
+      const int h = t2 >= CS, e = t2 - h * CS;
+      const int b = h ? f0.z : f0.y;
+      in.dt = mats_dt[(((size_t)k * a.B + b) * C + (e >> 4)) * 16 + (e & 15)];
+    }
+    {
+      const bool gpop = (f0.w & 3) == 2 && (f0.w >> 8) >= RS;
+      const int off = gpop ? off_real : off_hot;
+#pragma unroll
+      for (int c = 0; c < CM; ++c)
+        if (EXACT || c < C) in.Up[c] = ld4((gpop ? row(f0.x, c) : hot) + off);
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int ch = h ? f0.z : f0.y;
+      const bool tip = (f1.z >> h) & 1;
+      const int off = tip ? off_hot : off_real;
+#pragma unroll
+      for (int c = 0; c < CM; ++c)
+        if (EXACT || c < C) in.L[h][c] = ld4((tip ? hot : row(ch, c)) + off);
+      if constexpr (TIPS == HX_TIPS_CODES) {
+        in.code[h] = a.codes_pad[(size_t)(tip ? ch : 0) * a.np + (size_t)tl * 64 + lane];
+      } else {
+        const int nn = n < a.N ? n : a.N - 1;
+        in.tipv[h] = ld4(tip ? a.dense + ((size_t)ch * a.N + nn) * 4 : hot + off_hot);
+      }
+    }
+  };
+
+  // register stack: depths 0 and 1 (RS of them) in s0 and s1, moved by
+  // selects on the wave-uniform depth
+  static_assert(RS >= 0 && RS <= 2, "at most two register slots");
+  V4<R> U[CM], s0[CM], s1[CM];
+#pragma unroll
+  for (int c = 0; c < CM; ++c) U[c] = s0[c] = s1[c] = zero4<R>();
+  float* xs[2] = {xt, xt + 32 * kXS};             // scaled Ut rows of child h
+  float* ls[2] = {xt + 16 * kXS, xt + 48 * kXS};  // L rows of child h
+  const int r16 = lane & 15, g4 = lane >> 4;
+
+  // One entry per trip. Part 1 consumes the prefetched data (the adjoint, m,
+  // the statistic's transposes, a pushed adjoint's store); then the next
+  // entry's loads are issued into the same registers; part 2 forms the
+  // statistic on the matrix cores, sums it over the block's tiles and forms
+  // the continuing adjoint. Vector memory counters complete in order, so a
+  // store only delays a later wait when it is issued shortly before the
+  // awaited loads: the push store is issued early in part 1, and the
+  // statistic store (part 2, after the loads) is issued by every thread on
+  // every path so the wait for the next entry never has to cover it.
+  // The statistic of thread t: entry t mod 2CS of child h's G.
+  const int CS = C * 16, t2 = threadIdx.x % (2 * CS);
+  const int sh = t2 >= CS, se = t2 - sh * CS;
+  auto stat_store = [&](const int4& q0, float s, float dt) {
+    const int b = sh ? q0.z : q0.y;
+    if constexpr (TAU) {
+      double tv = (double)s * (double)dt;
+#pragma unroll
+      for (int o = 1; o < 16; o <<= 1) tv += __shfl_xor(tv, o);
+      a.gtau[(((size_t)b * a.K + k) * a.nchunks + chunk) * C + (se >> 4)] = tv;
+    } else {
+      a.gpart[(((size_t)b * a.K + k) * a.nchunks + chunk) * CS + se] = s;
+    }
+  };
+  PreIn<CM> cur;
+  int4 e0, e1;
+  entry(0, e0, e1);
+  fetch(e0, e1, cur);
+  // the store every trip ends with, here a zero that entry 0 overwrites, so
+  // the loop is entered with the same outstanding memory operations
+  stat_store(e0, 0.f, 0.f);
+  for (int i = 0; i < nprog; ++i) {
+    int4 f0 = e0, f1 = e1;
+    if (i + 1 < nprog) entry(i + 1, f0, f1);
+    const int ch[2] = {e0.y, e0.z};
+    const int usrc = e0.w, cont = e1.x, push = e1.y, tips = e1.z;
+
+    // ---------------------------------------------------------------- part 1
+    // the parent's adjoint: the frequencies, the previous entry's
+    // continuation (already in U) or a pop
+    if (usrc == 0) {
+#pragma unroll
+      for (int c = 0; c < CM; ++c) U[c] = pi;
+    } else if (usrc != 1) {
+      const int d = usrc >> 8;
+      if (d >= RS) {
+#pragma unroll
+        for (int c = 0; c < CM; ++c) U[c] = cur.Up[c];
+      } else {
+#pragma unroll
+        for (int c = 0; c < CM; ++c)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) U[c].x[q] = d == 0 ? s0[c].x[q] : s1[c].x[q];
+      }
+    }
+    // m_h = P_h L_h; L_h also goes to LDS as the statistic's B operand
+    V4<R> m[2][CM];
+    auto child = [&](auto hc) {
+      constexpr int h = decltype(hc)::value;
+      if ((tips >> h) & 1) {
+        V4<R> t;
+        if constexpr (TIPS == HX_TIPS_CODES) t = tbl[cur.code[h]];
+        else t = cur.tipv[h];
+#pragma unroll
+        for (int c = 0; c < CM; ++c) {
+          if (EXACT || c < C) {
+            R P[16];
+            load_p16(mk + ((size_t)ch[h] * C + c) * 16, P);
+            m[h][c] = mv(P, t);
+          }
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s) ls[h][s * kXS + lane] = t.x[s];
+      } else {
+#pragma unroll
+        for (int c = 0; c < CM; ++c) {
+          if (EXACT || c < C) {
+            R P[16];
+            load_p16(mk + ((size_t)ch[h] * C + c) * 16, P);
+            m[h][c] = mv(P, cur.L[h][c]);
+#pragma unroll
+            for (int s = 0; s < 4; ++s) ls[h][(4 * c + s) * kXS + lane] = cur.L[h][c].x[s];
+          }
+        }
+      }
+    };
+    child(H0{});
+    child(H1{});
+    float dt = 0.f;
+    if constexpr (TAU) dt = cur.dt;
+    R lik = R(0);
+#pragma unroll
+    for (int c = 0; c < CM; ++c) {
+      if (EXACT || c < C) {
+        R t = R(0);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) t = fma(U[c].x[s] * m[0][c].x[s], m[1][c].x[s], t);
+        lik = fma(pc[c], t, lik);
+      }
+    }
+    const R coef = (val && lik > R(0)) ? w / lik : R(0);
+    V4<R> Ut[2][CM];
+#pragma unroll
+    for (int c = 0; c < CM; ++c) {
+      if (EXACT || c < C) {
+        const R cw = coef * pc[c];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          Ut[0][c].x[s] = U[c].x[s] * m[1][c].x[s];
+          Ut[1][c].x[s] = U[c].x[s] * m[0][c].x[s];
+          xs[0][(4 * c + s) * kXS + lane] = cw * Ut[0][c].x[s];
+          xs[1][(4 * c + s) * kXS + lane] = cw * Ut[1][c].x[s];
+        }
+      }
+    }
+    // adjoints of internal children, computed straight into their registers
+    // (the child index is a compile-time constant in each branch)
+    auto adjoint = [&](auto hc, V4<R>(&Un)[CM]) {
+      constexpr int h = decltype(hc)::value;
+      R M = R(0);
+#pragma unroll
+      for (int c = 0; c < CM; ++c) {
+        if (EXACT || c < C) {
+          R P[16];
+          load_p16(mk + ((size_t)ch[h] * C + c) * 16, P);
+          Un[c] = mv_t(P, Ut[h][c]);
+          M = fmax_(M, max4(Un[c]));
+        } else {
+          Un[c] = zero4<R>();
+        }
+      }
+      const int ex = scale_exp(M);
+#pragma unroll
+      for (int c = 0; c < CM; ++c) scale4(Un[c], ex);
+    };
+    auto push_adjoint = [&](auto hc, int d) {
+      constexpr int h = decltype(hc)::value;
+      V4<R> Un[CM];
+      adjoint(hc, Un);
+      if (d >= RS) {
+        if (tval) {
+#pragma unroll
+          for (int c = 0; c < CM; ++c)
+            if (EXACT || c < C) st4(row(ch[h], c) + off_real, Un[c]);
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < CM; ++c)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            s0[c].x[q] = d == 0 ? Un[c].x[q] : s0[c].x[q];
+            s1[c].x[q] = d == 1 ? Un[c].x[q] : s1[c].x[q];
+          }
+      }
+    };
+    if (push >= 0) {
+      if ((push & 0xff) == 0) push_adjoint(H0{}, push >> 8);
+      else push_adjoint(H1{}, push >> 8);
+    }
+
+    // ------------------------------------------------ the next entry's loads
+    fetch(f0, f1, cur);
+
+    // ---------------------------------------------------------------- part 2
+    __builtin_amdgcn_wave_barrier();
+    f4 gacc[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) gacc[h] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int rl = ((tips >> h) & 1) ? (r16 & 3) : r16;
+        const f4 A = *reinterpret_cast<const f4*>(xs[h] + r16 * kXS + 16 * g4 + 4 * u);
+        const f4 Bv = *reinterpret_cast<const f4*>(ls[h] + rl * kXS + 16 * g4 + 4 * u);
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          gacc[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[e], Bv[e], gacc[h], 0, 0, 0);
+      }
+    }
+    // lane 20 c + j holds G_h[c][i][j] in register i
+    const int par = i & 1;
+    if ((lane & 15) >> 2 == g4 && g4 < C) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        float* dst = red + ((par * 2 + h) * W + wave) * 64 + g4 * 16 + (lane & 3);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) dst[4 * q] = gacc[h][q];
+      }
+    }
+    lds_barrier();
+    {
+      // thread t sums its entry over the W tiles; threads with the same entry
+      // store the same value to the same address
+      float s = 0.f;
+      for (int q = 0; q < W; ++q) s += red[((par * 2 + sh) * W + q) * 64 + se];
+      stat_store(e0, s, dt);
+    }
+    if (cont == 0) adjoint(H0{}, U);
+    else if (cont == 1) adjoint(H1{}, U);
+    e0 = f0;
+    e1 = f1;
+  }
+}
+
+template <int CM, bool EXACT, int RS>
+static void pre_walk_kernel(int tips, bool tau,
+                            void (*&kern)(PassArgs<float>, const float*, const float*, const int*,
+                                          int, int)) {
+  if (tau)
+    kern = tips == HX_TIPS_CODES ? k_pre_walk<HX_TIPS_CODES, CM, EXACT, RS, true>
+                                 : k_pre_walk<HX_TIPS_DENSE, CM, EXACT, RS, true>;
+  else
+    kern = tips == HX_TIPS_CODES ? k_pre_walk<HX_TIPS_CODES, CM, EXACT, RS, false>
+                                 : k_pre_walk<HX_TIPS_DENSE, CM, EXACT, RS, false>;
+}
+
 // ------------------------------------------------------------------ launchers
 static size_t tbl_bytes(int tips, int ncodes, size_t rb) {
   return tips == HX_TIPS_CODES ? (size_t)ncodes * 4 * rb : 0;
@@ -649,6 +996,31 @@ int launch_pre_dna(const PassArgs<R>& a, const int32_t* nodes, int width, int ti
   }
 #undef HX_PRE_T
 #undef HX_PRE
+  HX_HIP_TRY(hipGetLastError());
+  return HX_OK;
+}
+
+int launch_pre_walk(const PassArgs<float>& a, const int32_t* prog, int nprog, int tips,
+                    hipStream_t st) {
+  const int W = kPreWalkW;
+  HX_REQUIRE(a.C <= kPreWalkMaxC && a.nchunks == (a.ntiles + W - 1) / W, HX_ERR_UNSUPPORTED,
+             "pre-order walk: chunking does not match the workgroup's tiles");
+  const size_t lds = (size_t)W * 4 * 16 * kXS * sizeof(float) + (size_t)4 * W * 64 * sizeof(float) +
+                     tbl_bytes(tips, a.ncodes, sizeof(float));
+  HX_REQUIRE(lds <= kDnaWalkLdsMax, HX_ERR_UNSUPPORTED, "pre-order walk: LDS");
+  void (*kern)(PassArgs<float>, const float*, const float*, const int*, int, int) = nullptr;
+  constexpr int RS = 2;
+  const bool tau = a.mats_dt != nullptr;
+  switch (a.C) {
+    case 1: pre_walk_kernel<1, true, RS>(tips, tau, kern); break;
+    case 2: pre_walk_kernel<2, true, RS>(tips, tau, kern); break;
+    case 3: pre_walk_kernel<4, false, RS>(tips, tau, kern); break;
+    default: pre_walk_kernel<4, true, RS>(tips, tau, kern); break;
+  }
+  if (lds > 64 * 1024)
+    HX_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  kern<<<dim3(a.nchunks, a.K), 64 * W, lds, st>>>(a, a.mats, a.mats_dt, prog, nprog, W);
   HX_HIP_TRY(hipGetLastError());
   return HX_OK;
 }

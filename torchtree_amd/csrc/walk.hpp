@@ -121,3 +121,70 @@ inline bool build_walk(int T, const std::vector<int32_t>& kids, const int32_t* p
 }
 
 }  // namespace hx
+
+namespace hx {
+
+// Pre-order walk program: one entry per internal node, depth first from the
+// root. An entry processes parent p with children a and b given p's adjoint U
+// (root: the frequencies): it forms both children's statistics and adjoints.
+// One internal child continues the walk (its adjoint stays in registers for
+// the next entry); when both are internal the one with the smaller subtree
+// continues and the other's adjoint is pushed, to be popped when the walk
+// below the continuing child is finished. A push at depth d waits for that
+// whole subtree, so shallow depths (long waits) are the ones worth keeping in
+// registers. Continuing into the smaller subtree bounds the depth by
+// floor(log2 T). kPreInts ints per entry:
+//   [0] p  [1] a  [2] b
+//   [3] usrc = 0 root, 1 continuation of the previous entry, 2 | d << 8 pop
+//       of depth d
+//   [4] cont = child (0 a, 1 b) whose adjoint the next entry uses, or -1
+//   [5] push = child | d << 8 whose adjoint is pushed at depth d, or -1
+//   [6] tips = (a is a tip) | (b is a tip) << 1
+//   [7] 0
+constexpr int kPreInts = 8;
+
+inline bool build_pre_walk(int T, const std::vector<int32_t>& kids, const int32_t* postorder,
+                           std::vector<int32_t>& prog, int& depth) {
+  const int I = T - 1, nn = 2 * T - 1;
+  const int root = postorder[3 * (I - 1)];
+  std::vector<int> size(nn, 0);  // internal nodes in the subtree
+  for (int r = 0; r < I; ++r) {
+    const int v = postorder[3 * r], a = kids[2 * v], b = kids[2 * v + 1];
+    size[v] = 1 + (a >= T ? size[a] : 0) + (b >= T ? size[b] : 0);
+  }
+  prog.clear();
+  prog.reserve((size_t)kPreInts * I);
+  depth = 0;
+  std::vector<int> pend;  // pushed nodes, deepest last
+  int cur = root, usrc = 0;
+  for (int guard = 0; guard < I; ++guard) {
+    const int a = kids[2 * cur], b = kids[2 * cur + 1];
+    const bool ia = a >= T, ib = b >= T;
+    int cont = -1, push = -1;
+    if (ia && ib) {
+      cont = size[a] <= size[b] ? 0 : 1;
+      push = (1 - cont) | ((int)pend.size() << 8);
+      pend.push_back(cont == 0 ? b : a);
+      depth = std::max(depth, (int)pend.size());
+    } else if (ia) {
+      cont = 0;
+    } else if (ib) {
+      cont = 1;
+    }
+    const int32_t e[kPreInts] = {cur, a, b, usrc, cont, push, (a < T ? 1 : 0) | (b < T ? 2 : 0), 0};
+    prog.insert(prog.end(), e, e + kPreInts);
+    if (cont >= 0) {
+      cur = cont == 0 ? a : b;
+      usrc = 1;
+    } else if (!pend.empty()) {
+      cur = pend.back();
+      pend.pop_back();
+      usrc = 2 | ((int)pend.size() << 8);
+    } else {
+      break;
+    }
+  }
+  return (int)prog.size() == kPreInts * I && prog[0] == root && pend.empty() && depth <= 255;
+}
+
+}  // namespace hx
