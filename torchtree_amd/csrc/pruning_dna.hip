@@ -611,10 +611,10 @@ __global__ __launch_bounds__(256) void k_pre_walk(PassArgs<float> a, const float
   const int tl = tval ? tile : a.ntiles - 1;
   const int n = tl * 64 + lane;
   const bool val = tval && n < a.N;
-  float* xt = reinterpret_cast<float*>(smem_raw) + (size_t)wave * 4 * 16 * kXS;  // [4][16][kXS]
-  float* red = reinterpret_cast<float*>(smem_raw) + (size_t)W * 4 * 16 * kXS;    // [2][2][W][64]
+  float* xt = reinterpret_cast<float*>(smem_raw) + (size_t)wave * 2 * 16 * kXS;  // [2][16][kXS]
+  float* red = reinterpret_cast<float*>(smem_raw) + (size_t)W * 2 * 16 * kXS;    // [2][2][W][64]
   V4<R>* tbl = reinterpret_cast<V4<R>*>(red + 4 * W * 64);
-  for (int i = lane; i < 4 * 16 * kXS; i += 64) xt[i] = 0.f;  // rows of absent categories
+  for (int i = lane; i < 2 * 16 * kXS; i += 64) xt[i] = 0.f;  // rows of absent categories
   stage_tbl<R, TIPS>(a, tbl, blockDim.x);
   __syncthreads();
 
@@ -630,11 +630,13 @@ __global__ __launch_bounds__(256) void k_pre_walk(PassArgs<float> a, const float
   // lane: the (node, category) row of this tile, or for placeholders one
   // cache line of this sample's P blocks that every wave keeps reading.
   const size_t cstride = (size_t)a.np * 4;
-  auto row = [&](int node, int c) -> R* {
-    return a.partials + (((size_t)(node - a.T) * a.K + k) * C + c) * cstride + (size_t)tl * 256;
-  };
   const R* hot = mats + (size_t)k * a.B * C * 16;
-  const int off_real = lane * 4, off_hot = (lane & 3) * 4;
+  // lane byte offsets, unsigned so the loads use a scalar base plus a 32-bit
+  // vector offset
+  const unsigned off_real = lane * 16u, off_hot = (lane & 3) * 16u;
+  auto at = [](const R* base, unsigned off) {
+    return reinterpret_cast<const R*>(reinterpret_cast<const char*>(base) + off);
+  };
   const cptr<R> mk = to_const(mats) + (size_t)k * a.B * C * 16;
   const cptr<int> cprog = to_const(prog);
   auto entry = [&](int i, int4& f0, int4& f1) {
@@ -645,6 +647,11 @@ __global__ __launch_bounds__(256) void k_pre_walk(PassArgs<float> a, const float
   // Every entry issues the same vector loads (placeholders where the entry
   // has nothing to read), so the wait for one entry's data counts the next
   // entry's loads exactly and never drains them.
+  // Uniform base of a node's rows for this tile and sample; a child or pop
+  // with nothing to read uses the hot line with a category stride of zero.
+  const R* wbase = a.partials + (size_t)k * C * cstride + (size_t)tl * 256;
+  const size_t sstride = (size_t)a.K * C * cstride;
+  const uint8_t* cbase = a.codes_pad + (size_t)tl * 64 + lane;
   auto fetch = [&](const int4& f0, const int4& f1, PreIn<CM>& in) {
     if constexpr (TAU) {
       // issued first: the copy of this value into the loop-carried register
@@ -656,27 +663,32 @@ __global__ __launch_bounds__(256) void k_pre_walk(PassArgs<float> a, const float
     }
     {
       const bool gpop = (f0.w & 3) == 2 && (f0.w >> 8) >= RS;
-      const int off = gpop ? off_real : off_hot;
+      const R* pb = gpop ? wbase + (size_t)(f0.x - a.T) * sstride : hot;
+      const size_t cs = gpop ? cstride : 0;
+      const unsigned off = gpop ? off_real : off_hot;
 #pragma unroll
       for (int c = 0; c < CM; ++c)
-        if (EXACT || c < C) in.Up[c] = ld4((gpop ? row(f0.x, c) : hot) + off);
+        if (EXACT || c < C) in.Up[c] = ld4(at(pb + c * cs, off));
     }
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int ch = h ? f0.z : f0.y;
       const bool tip = (f1.z >> h) & 1;
-      const int off = tip ? off_hot : off_real;
+      const R* pb = tip ? hot : wbase + (size_t)(ch - a.T) * sstride;
+      const size_t cs = tip ? 0 : cstride;
+      const unsigned off = tip ? off_hot : off_real;
 #pragma unroll
       for (int c = 0; c < CM; ++c)
-        if (EXACT || c < C) in.L[h][c] = ld4((tip ? hot : row(ch, c)) + off);
+        if (EXACT || c < C) in.L[h][c] = ld4(at(pb + c * cs, off));
       if constexpr (TIPS == HX_TIPS_CODES) {
-        in.code[h] = a.codes_pad[(size_t)(tip ? ch : 0) * a.np + (size_t)tl * 64 + lane];
+        in.code[h] = cbase[(size_t)(tip ? ch : 0) * a.np];
       } else {
         const int nn = n < a.N ? n : a.N - 1;
-        in.tipv[h] = ld4(tip ? a.dense + ((size_t)ch * a.N + nn) * 4 : hot + off_hot);
+        in.tipv[h] = ld4(tip ? a.dense + ((size_t)ch * a.N + nn) * 4 : at(hot, off_hot));
       }
     }
   };
+
 
   // register stack: depths 0 and 1 (RS of them) in s0 and s1, moved by
   // selects on the wave-uniform depth
@@ -684,8 +696,8 @@ __global__ __launch_bounds__(256) void k_pre_walk(PassArgs<float> a, const float
   V4<R> U[CM], s0[CM], s1[CM];
 #pragma unroll
   for (int c = 0; c < CM; ++c) U[c] = s0[c] = s1[c] = zero4<R>();
-  float* xs[2] = {xt, xt + 32 * kXS};             // scaled Ut rows of child h
-  float* ls[2] = {xt + 16 * kXS, xt + 48 * kXS};  // L rows of child h
+  float* xs = xt;              // rows 4c + i: (w p_c / lik) Ut[c][i]
+  float* ls = xt + 16 * kXS;    // rows 4c + j: L[c][j] (a tip: rows j)
   const int r16 = lane & 15, g4 = lane >> 4;
 
   // One entry per trip. Part 1 consumes the prefetched data (the adjoint, m,
@@ -742,24 +754,21 @@ __global__ __launch_bounds__(256) void k_pre_walk(PassArgs<float> a, const float
           for (int q = 0; q < 4; ++q) U[c].x[q] = d == 0 ? s0[c].x[q] : s1[c].x[q];
       }
     }
-    // m_h = P_h L_h; L_h also goes to LDS as the statistic's B operand
-    V4<R> m[2][CM];
+    // m_h = P_h L_h
+    V4<R> m[2][CM], tv[2];
     auto child = [&](auto hc) {
       constexpr int h = decltype(hc)::value;
       if ((tips >> h) & 1) {
-        V4<R> t;
-        if constexpr (TIPS == HX_TIPS_CODES) t = tbl[cur.code[h]];
-        else t = cur.tipv[h];
+        if constexpr (TIPS == HX_TIPS_CODES) tv[h] = tbl[cur.code[h]];
+        else tv[h] = cur.tipv[h];
 #pragma unroll
         for (int c = 0; c < CM; ++c) {
           if (EXACT || c < C) {
             R P[16];
             load_p16(mk + ((size_t)ch[h] * C + c) * 16, P);
-            m[h][c] = mv(P, t);
+            m[h][c] = mv(P, tv[h]);
           }
         }
-#pragma unroll
-        for (int s = 0; s < 4; ++s) ls[h][s * kXS + lane] = t.x[s];
       } else {
 #pragma unroll
         for (int c = 0; c < CM; ++c) {
@@ -767,8 +776,6 @@ __global__ __launch_bounds__(256) void k_pre_walk(PassArgs<float> a, const float
             R P[16];
             load_p16(mk + ((size_t)ch[h] * C + c) * 16, P);
             m[h][c] = mv(P, cur.L[h][c]);
-#pragma unroll
-            for (int s = 0; s < 4; ++s) ls[h][(4 * c + s) * kXS + lane] = cur.L[h][c].x[s];
           }
         }
       }
@@ -792,16 +799,54 @@ __global__ __launch_bounds__(256) void k_pre_walk(PassArgs<float> a, const float
 #pragma unroll
     for (int c = 0; c < CM; ++c) {
       if (EXACT || c < C) {
-        const R cw = coef * pc[c];
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
           Ut[0][c].x[s] = U[c].x[s] * m[1][c].x[s];
           Ut[1][c].x[s] = U[c].x[s] * m[0][c].x[s];
-          xs[0][(4 * c + s) * kXS + lane] = cw * Ut[0][c].x[s];
-          xs[1][(4 * c + s) * kXS + lane] = cw * Ut[1][c].x[s];
         }
       }
     }
+    // The statistic of child h: its rows to the wave's LDS transposes, then
+    // 16 products on the matrix cores; child 1's rows overwrite child 0's
+    // after child 0's operand reads (LDS operations of a wave stay in order).
+    auto stat_rows = [&](auto hc) {
+      constexpr int h = decltype(hc)::value;
+      const bool tip = (tips >> h) & 1;
+#pragma unroll
+      for (int c = 0; c < CM; ++c) {
+        if (EXACT || c < C) {
+          const R cw = coef * pc[c];
+#pragma unroll
+          for (int s = 0; s < 4; ++s) xs[(4 * c + s) * kXS + lane] = cw * Ut[h][c].x[s];
+          if (!tip) {
+#pragma unroll
+            for (int s = 0; s < 4; ++s) ls[(4 * c + s) * kXS + lane] = cur.L[h][c].x[s];
+          }
+        }
+      }
+      if (tip) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) ls[s * kXS + lane] = tv[h].x[s];
+      }
+    };
+    auto stat_mfma = [&](int h) {
+      __builtin_amdgcn_wave_barrier();
+      const int rl = ((tips >> h) & 1) ? (r16 & 3) : r16;
+      f4 g = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const f4 A = *reinterpret_cast<const f4*>(xs + r16 * kXS + 16 * g4 + 4 * u);
+        const f4 Bv = *reinterpret_cast<const f4*>(ls + rl * kXS + 16 * g4 + 4 * u);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) g = __builtin_amdgcn_mfma_f32_16x16x4f32(A[e], Bv[e], g, 0, 0, 0);
+      }
+      __builtin_amdgcn_wave_barrier();
+      return g;
+    };
+    f4 gacc[2];
+    stat_rows(H0{});
+    gacc[0] = stat_mfma(0);
+    stat_rows(H1{});
     // adjoints of internal children, computed straight into their registers
     // (the child index is a compile-time constant in each branch)
     auto adjoint = [&](auto hc, V4<R>(&Un)[CM]) {
@@ -828,9 +873,10 @@ __global__ __launch_bounds__(256) void k_pre_walk(PassArgs<float> a, const float
       adjoint(hc, Un);
       if (d >= RS) {
         if (tval) {
+          R* dst = const_cast<R*>(wbase) + (size_t)(ch[h] - a.T) * sstride;
 #pragma unroll
           for (int c = 0; c < CM; ++c)
-            if (EXACT || c < C) st4(row(ch[h], c) + off_real, Un[c]);
+            if (EXACT || c < C) st4(const_cast<R*>(at(dst + c * cstride, off_real)), Un[c]);
         }
       } else {
 #pragma unroll
@@ -846,27 +892,14 @@ __global__ __launch_bounds__(256) void k_pre_walk(PassArgs<float> a, const float
       if ((push & 0xff) == 0) push_adjoint(H0{}, push >> 8);
       else push_adjoint(H1{}, push >> 8);
     }
+    if (cont == 0) adjoint(H0{}, U);
+    else if (cont == 1) adjoint(H1{}, U);
 
     // ------------------------------------------------ the next entry's loads
     fetch(f0, f1, cur);
 
     // ---------------------------------------------------------------- part 2
-    __builtin_amdgcn_wave_barrier();
-    f4 gacc[2];
-#pragma unroll
-    for (int h = 0; h < 2; ++h) gacc[h] = f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int rl = ((tips >> h) & 1) ? (r16 & 3) : r16;
-        const f4 A = *reinterpret_cast<const f4*>(xs[h] + r16 * kXS + 16 * g4 + 4 * u);
-        const f4 Bv = *reinterpret_cast<const f4*>(ls[h] + rl * kXS + 16 * g4 + 4 * u);
-#pragma unroll
-        for (int e = 0; e < 4; ++e)
-          gacc[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[e], Bv[e], gacc[h], 0, 0, 0);
-      }
-    }
+    gacc[1] = stat_mfma(1);
     // lane 20 c + j holds G_h[c][i][j] in register i
     const int par = i & 1;
     if ((lane & 15) >> 2 == g4 && g4 < C) {
@@ -885,8 +918,6 @@ __global__ __launch_bounds__(256) void k_pre_walk(PassArgs<float> a, const float
       for (int q = 0; q < W; ++q) s += red[((par * 2 + sh) * W + q) * 64 + se];
       stat_store(e0, s, dt);
     }
-    if (cont == 0) adjoint(H0{}, U);
-    else if (cont == 1) adjoint(H1{}, U);
     e0 = f0;
     e1 = f1;
   }
@@ -1005,7 +1036,7 @@ int launch_pre_walk(const PassArgs<float>& a, const int32_t* prog, int nprog, in
   const int W = kPreWalkW;
   HX_REQUIRE(a.C <= kPreWalkMaxC && a.nchunks == (a.ntiles + W - 1) / W, HX_ERR_UNSUPPORTED,
              "pre-order walk: chunking does not match the workgroup's tiles");
-  const size_t lds = (size_t)W * 4 * 16 * kXS * sizeof(float) + (size_t)4 * W * 64 * sizeof(float) +
+  const size_t lds = (size_t)W * 2 * 16 * kXS * sizeof(float) + (size_t)4 * W * 64 * sizeof(float) +
                      tbl_bytes(tips, a.ncodes, sizeof(float));
   HX_REQUIRE(lds <= kDnaWalkLdsMax, HX_ERR_UNSUPPORTED, "pre-order walk: LDS");
   void (*kern)(PassArgs<float>, const float*, const float*, const int*, int, int) = nullptr;
