@@ -76,15 +76,24 @@ def walk_need(post, T):
     return need[post[-1][0]]
 
 
+def pre_walk_used(S, C, rb):
+    """The four-state FP32 pre-order runs as one walk (mirrors use_pre_walk, csrc/common.hpp)."""
+    return S == 4 and rb == 4 and C <= 4
+
+
 def algorithmic_bytes(T, N, C, K, S, rb, nchunks):
     """Bytes each pass must move at minimum with this algorithm (one launch set).
 
     P = K N C S rb is one node's partials. Four states: the post-order walk
     writes every internal non-root partial once and reads none back from HBM
-    (a child that waited on the stack is re-read from L2), plus the tip codes;
-    the pre-order reads each internal non-root node's L and its parent's U and
-    writes its U (3 P per node). Other state counts use level kernels: post
-    2 P per node (written, read by the parent) plus the exponents.
+    (a child that waited on the stack is re-read from L2), plus the tip codes.
+    The four-state FP32 pre-order walk reads every internal non-root partial
+    once (the adjoints live in registers), reads the tip codes once per sample
+    and writes the per-chunk branch statistics, which the reduction reads
+    back. Otherwise the pre-order runs per level: each internal non-root node's
+    L and its parent's U are read and its U written (3 P per node). Other state
+    counts use level kernels: post 2 P per node (written, read by the parent)
+    plus the exponents.
     """
     part = K * N * C * S * rb
     I = T - 1
@@ -96,7 +105,10 @@ def algorithmic_bytes(T, N, C, K, S, rb, nchunks):
         post_b = (I - 1) * part + tips + mats + weights
     else:
         post_b = 2 * (I - 1) * part + 2 * (I - 1) * K * N * 4 + tips + mats + weights
-    pre_b = 3 * (I - 1) * part + tips + mats + weights + gpart
+    if pre_walk_used(S, C, rb):
+        pre_b = (I - 1) * part + K * tips + mats + weights + 2 * gpart
+    else:
+        pre_b = 3 * (I - 1) * part + tips + mats + weights + gpart
     return post_b, pre_b
 
 
@@ -220,11 +232,16 @@ def main():
         "phases_ms": {k: v["ms"] for k, v in phases.items()},
         "roofline": {
             "bound": "hbm",
-            "kernel": (f"k_pre ({launches - 1} depth-level launches) + k_reduce_g"
+            "kernel": (("k_pre_walk (one launch, whole tree) + k_reduce_g"
+                        if pre_walk_used(S, wl.C, 4) else
+                        f"k_pre ({launches - 1} depth-level launches) + k_reduce_g")
                        if dom == "preorder" else "k_post_walk (one launch, whole tree) + k_reduce_root"),
             "achieved": dom_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dom_gbs / HBM_PEAK_GBS,
             "traffic": measured_traffic(args, dom), "algorithmic_bytes": dom_b, "pass_ms": dom_ms,
-            "avg_launch_us": dom_ms * 1e3 / max(1, launches - 1),
+            # the walks are one launch for the whole tree (pass time includes the
+            # small reduction that follows); level kernels one launch per level
+            "avg_launch_us": dom_ms * 1e3 / (1 if (dom == "postorder" or pre_walk_used(S, wl.C, 4))
+                                             else max(1, launches - 1)),
         },
         "other_pass": {"postorder_gbs": post_gbs, "preorder_gbs": pre_gbs,
                        "postorder_bytes": post_b, "preorder_bytes": pre_b,
@@ -372,10 +389,12 @@ def measured_traffic(args, dom):
     return d["read_bytes"] + d["write_bytes"]
 
 
-def _nchunks(N, C, K):
+def _nchunks(N, C, K, rb=4):
     """Chunk count of the S=4 kernels (mirrors make_tiling in csrc/common.hpp)."""
     ppb = 64
     ntiles = (N + ppb - 1) // ppb
+    if pre_walk_used(4, C, rb):
+        return (ntiles + 3) // 4  # four tiles per pre-order walk workgroup
     nch = max(1, min(ntiles, 128))
     tpc = (ntiles + nch - 1) // nch
     return (ntiles + tpc - 1) // tpc
