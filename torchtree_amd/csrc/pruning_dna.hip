@@ -56,6 +56,24 @@ __device__ __forceinline__ void st4(R* p, const V4<R>& v) {
   }
 }
 
+// Non-temporal load (streaming hint: the line is a first candidate for
+// eviction), for data read once so that lines re-read soon stay in L2.
+template <typename R>
+__device__ __forceinline__ V4<R> ld4_nt(const R* p) {
+  V4<R> v;
+  if constexpr (sizeof(R) == 4) {
+    using f4 = float __attribute__((ext_vector_type(4)));
+    const f4 q = __builtin_nontemporal_load(reinterpret_cast<const f4*>(p));
+    v.x[0] = q[0], v.x[1] = q[1], v.x[2] = q[2], v.x[3] = q[3];
+  } else {
+    using d2 = double __attribute__((ext_vector_type(2)));
+    const d2 q0 = __builtin_nontemporal_load(reinterpret_cast<const d2*>(p));
+    const d2 q1 = __builtin_nontemporal_load(reinterpret_cast<const d2*>(p) + 1);
+    v.x[0] = q0[0], v.x[1] = q0[1], v.x[2] = q1[0], v.x[3] = q1[1];
+  }
+  return v;
+}
+
 template <typename R>
 __device__ __forceinline__ V4<R> zero4() {
   V4<R> v;
@@ -594,7 +612,8 @@ struct PreIn {
 };
 
 template <int TIPS, int CM, bool EXACT, int RS, bool TAU>
-__global__ __launch_bounds__(256) void k_pre_walk(PassArgs<float> a, const float* __restrict__ mats,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
+    !EXACT || (TAU && TIPS == HX_TIPS_CODES) ? 1 : (RS == 0 ? 4 : 3)))) void k_pre_walk(PassArgs<float> a, const float* __restrict__ mats,
                                                   const float* __restrict__ mats_dt,
                                                   const int* __restrict__ prog, int nprog, int W) {
   using R = float;
@@ -677,9 +696,11 @@ __global__ __launch_bounds__(256) void k_pre_walk(PassArgs<float> a, const float
       const R* pb = tip ? hot : wbase + (size_t)(ch - a.T) * sstride;
       const size_t cs = tip ? 0 : cstride;
       const unsigned off = tip ? off_hot : off_real;
+      // a post-order partial is read once: streaming, so that pushed
+      // adjoints stay in L2 until their pop
 #pragma unroll
       for (int c = 0; c < CM; ++c)
-        if (EXACT || c < C) in.L[h][c] = ld4(at(pb + c * cs, off));
+        if (EXACT || c < C) in.L[h][c] = ld4_nt(at(pb + c * cs, off));
       if constexpr (TIPS == HX_TIPS_CODES) {
         in.code[h] = cbase[(size_t)(tip ? ch : 0) * a.np];
       } else {
@@ -712,6 +733,7 @@ __global__ __launch_bounds__(256) void k_pre_walk(PassArgs<float> a, const float
   // The statistic of thread t: entry t mod 2CS of child h's G.
   const int CS = C * 16, t2 = threadIdx.x % (2 * CS);
   const int sh = t2 >= CS, se = t2 - sh * CS;
+  const R spc = a.props[(size_t)k * a.props_ks + (se >> 4)];  // p_c of this thread's entry
   auto stat_store = [&](const int4& q0, float s, float dt) {
     const int b = sh ? q0.z : q0.y;
     if constexpr (TAU) {
@@ -784,28 +806,24 @@ __global__ __launch_bounds__(256) void k_pre_walk(PassArgs<float> a, const float
     child(H1{});
     float dt = 0.f;
     if constexpr (TAU) dt = cur.dt;
+    V4<R> Ut[2][CM];
     R lik = R(0);
 #pragma unroll
     for (int c = 0; c < CM; ++c) {
       if (EXACT || c < C) {
         R t = R(0);
 #pragma unroll
-        for (int s = 0; s < 4; ++s) t = fma(U[c].x[s] * m[0][c].x[s], m[1][c].x[s], t);
-        lik = fma(pc[c], t, lik);
-      }
-    }
-    const R coef = (val && lik > R(0)) ? w / lik : R(0);
-    V4<R> Ut[2][CM];
-#pragma unroll
-    for (int c = 0; c < CM; ++c) {
-      if (EXACT || c < C) {
-#pragma unroll
         for (int s = 0; s < 4; ++s) {
           Ut[0][c].x[s] = U[c].x[s] * m[1][c].x[s];
           Ut[1][c].x[s] = U[c].x[s] * m[0][c].x[s];
+          t = fma(Ut[0][c].x[s], m[0][c].x[s], t);
         }
+        lik = fma(pc[c], t, lik);
       }
     }
+    // w / lik goes into the B operand (L: 4 values for a tip) and p_c is
+    // applied to the summed statistic
+    const R coef = (val && lik > R(0)) ? w * __builtin_amdgcn_rcpf(lik) : R(0);
     // The statistic of child h: its rows to the wave's LDS transposes, then
     // 16 products on the matrix cores; child 1's rows overwrite child 0's
     // after child 0's operand reads (LDS operations of a wave stay in order).
@@ -815,18 +833,17 @@ __global__ __launch_bounds__(256) void k_pre_walk(PassArgs<float> a, const float
 #pragma unroll
       for (int c = 0; c < CM; ++c) {
         if (EXACT || c < C) {
-          const R cw = coef * pc[c];
 #pragma unroll
-          for (int s = 0; s < 4; ++s) xs[(4 * c + s) * kXS + lane] = cw * Ut[h][c].x[s];
+          for (int s = 0; s < 4; ++s) xs[(4 * c + s) * kXS + lane] = Ut[h][c].x[s];
           if (!tip) {
 #pragma unroll
-            for (int s = 0; s < 4; ++s) ls[(4 * c + s) * kXS + lane] = cur.L[h][c].x[s];
+            for (int s = 0; s < 4; ++s) ls[(4 * c + s) * kXS + lane] = coef * cur.L[h][c].x[s];
           }
         }
       }
       if (tip) {
 #pragma unroll
-        for (int s = 0; s < 4; ++s) ls[s * kXS + lane] = tv[h].x[s];
+        for (int s = 0; s < 4; ++s) ls[s * kXS + lane] = coef * tv[h].x[s];
       }
     };
     auto stat_mfma = [&](int h) {
@@ -916,23 +933,27 @@ __global__ __launch_bounds__(256) void k_pre_walk(PassArgs<float> a, const float
       // store the same value to the same address
       float s = 0.f;
       for (int q = 0; q < W; ++q) s += red[((par * 2 + sh) * W + q) * 64 + se];
-      stat_store(e0, s, dt);
+      stat_store(e0, s * spc, dt);
     }
     e0 = f0;
     e1 = f1;
   }
 }
 
-template <int CM, bool EXACT, int RS>
+// Occupancy and stack: the d loglik / d mats walk with tip codes fits 128
+// VGPRs with every push in memory (4 waves per SIMD); the other variants
+// keep two register stack slots, at 3 waves per SIMD (168 VGPRs) where that
+// needs no spill (a spill's scratch traffic would break the in-order waits).
+template <int CM, bool EXACT>
 static void pre_walk_kernel(int tips, bool tau,
                             void (*&kern)(PassArgs<float>, const float*, const float*, const int*,
                                           int, int)) {
   if (tau)
-    kern = tips == HX_TIPS_CODES ? k_pre_walk<HX_TIPS_CODES, CM, EXACT, RS, true>
-                                 : k_pre_walk<HX_TIPS_DENSE, CM, EXACT, RS, true>;
+    kern = tips == HX_TIPS_CODES ? k_pre_walk<HX_TIPS_CODES, CM, EXACT, 2, true>
+                                 : k_pre_walk<HX_TIPS_DENSE, CM, EXACT, 2, true>;
   else
-    kern = tips == HX_TIPS_CODES ? k_pre_walk<HX_TIPS_CODES, CM, EXACT, RS, false>
-                                 : k_pre_walk<HX_TIPS_DENSE, CM, EXACT, RS, false>;
+    kern = tips == HX_TIPS_CODES ? k_pre_walk<HX_TIPS_CODES, CM, EXACT, 0, false>
+                                 : k_pre_walk<HX_TIPS_DENSE, CM, EXACT, 2, false>;
 }
 
 // ------------------------------------------------------------------ launchers
@@ -1040,13 +1061,12 @@ int launch_pre_walk(const PassArgs<float>& a, const int32_t* prog, int nprog, in
                      tbl_bytes(tips, a.ncodes, sizeof(float));
   HX_REQUIRE(lds <= kDnaWalkLdsMax, HX_ERR_UNSUPPORTED, "pre-order walk: LDS");
   void (*kern)(PassArgs<float>, const float*, const float*, const int*, int, int) = nullptr;
-  constexpr int RS = 2;
   const bool tau = a.mats_dt != nullptr;
   switch (a.C) {
-    case 1: pre_walk_kernel<1, true, RS>(tips, tau, kern); break;
-    case 2: pre_walk_kernel<2, true, RS>(tips, tau, kern); break;
-    case 3: pre_walk_kernel<4, false, RS>(tips, tau, kern); break;
-    default: pre_walk_kernel<4, true, RS>(tips, tau, kern); break;
+    case 1: pre_walk_kernel<1, true>(tips, tau, kern); break;
+    case 2: pre_walk_kernel<2, true>(tips, tau, kern); break;
+    case 3: pre_walk_kernel<4, false>(tips, tau, kern); break;
+    default: pre_walk_kernel<4, true>(tips, tau, kern); break;
   }
   if (lds > 64 * 1024)
     HX_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
