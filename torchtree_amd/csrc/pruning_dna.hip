@@ -196,7 +196,11 @@ __device__ __forceinline__ void acc_outer(R (&g)[16], R w, const V4<R>& u, const
 // node's stores so the wait for it does not wait for them, and its exponent
 // comes from the lane's LDS stack. grid = (ntiles, ceil(K / W)), block = 64 W.
 // CM: categories rounded up to a power of two (EXACT: C == CM).
-template <typename R, int TIPS, int CM, bool EXACT>
+// NRS > 0: results at stack depths kRegDepth0 .. kRegDepth0 + NRS - 1 also
+// stay in registers, so their consumers do not read them back from memory.
+constexpr int kRegDepth0 = 2;
+
+template <typename R, int TIPS, int CM, bool EXACT, int NRS>
 __global__ __launch_bounds__(CM >= 8 ? 256 : 1024) void k_post_walk(PassArgs<R> a, const R* __restrict__ mats,
                                                     const int4* __restrict__ prog, int nprog,
                                                     int ne, int W) {
@@ -247,11 +251,17 @@ __global__ __launch_bounds__(CM >= 8 ? 256 : 1024) void k_post_walk(PassArgs<R> 
   // unused): every iteration then issues the same vector memory operations,
   // so the wait for these loads counts only the older ones and not this
   // node's stores, and the placeholder loads never reach HBM.
+  auto in_regs = [&](int slot) { return slot >= kRegDepth0 && slot < kRegDepth0 + NRS; };
   auto stacked_of = [&](const int4& f0, const int4& f1) {
-    if ((f1.z & 0xff) == 2) return f0.y;
-    if (((f1.z >> 8) & 0xff) == 2) return f0.z;
+    if ((f1.z & 0xff) == 2 && !in_regs((f0.w & 0xff) - 1)) return f0.y;
+    if (((f1.z >> 8) & 0xff) == 2 && !in_regs(((f0.w >> 8) & 0xff) - 1)) return f0.z;
     return -1;
   };
+  // register copies of the results at depths kRegDepth0 + 0, 1, 2
+  static_assert(NRS >= 0 && NRS <= 3, "at most three register depths");
+  V4<R> g0[CM], g1[CM], g2[CM];
+#pragma unroll
+  for (int c = 0; c < CM; ++c) g0[c] = g1[c] = g2[c] = zero4<R>();
   const R* hot = mats + (size_t)k * a.B * C * 16 + (lane & 3) * 4;
   auto readback = [&](int ch, V4<R>(&x)[CM]) {
     const bool real = ch >= 0;
@@ -286,8 +296,18 @@ __global__ __launch_bounds__(CM >= 8 ? 256 : 1024) void k_post_walk(PassArgs<R> 
       for (int c = 0; c < CM; ++c) xl[c] = y[c];
       El = E;
     } else {
+      if (in_regs(sl)) {
 #pragma unroll
-      for (int c = 0; c < CM; ++c) xl[c] = xs[c];
+        for (int c = 0; c < CM; ++c)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const R u = sl == kRegDepth0 + 1 ? g1[c].x[q] : g2[c].x[q];
+            xl[c].x[q] = sl == kRegDepth0 ? g0[c].x[q] : u;
+          }
+      } else {
+#pragma unroll
+        for (int c = 0; c < CM; ++c) xl[c] = xs[c];
+      }
       El = est[sl * 64 + lane];
     }
     if (kr == 0) {
@@ -299,8 +319,18 @@ __global__ __launch_bounds__(CM >= 8 ? 256 : 1024) void k_post_walk(PassArgs<R> 
       for (int c = 0; c < CM; ++c) xr[c] = y[c];
       Er = E;
     } else {
+      if (in_regs(sr)) {
 #pragma unroll
-      for (int c = 0; c < CM; ++c) xr[c] = xs[c];
+        for (int c = 0; c < CM; ++c)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const R u = sr == kRegDepth0 + 1 ? g1[c].x[q] : g2[c].x[q];
+            xr[c].x[q] = sr == kRegDepth0 ? g0[c].x[q] : u;
+          }
+      } else {
+#pragma unroll
+        for (int c = 0; c < CM; ++c) xr[c] = xs[c];
+      }
       Er = est[sr * 64 + lane];
     }
     // P of (l, c) and (r, c); the loads of category c + 1 are issued before
@@ -337,6 +367,16 @@ __global__ __launch_bounds__(CM >= 8 ? 256 : 1024) void k_post_walk(PassArgs<R> 
     for (int c = 0; c < CM; ++c)
       if (EXACT || c < C) st4(p + (size_t)c * cstride, y[c]);
     est[so * 64 + lane] = E;
+    if (NRS > 0) {
+#pragma unroll
+      for (int c = 0; c < CM; ++c)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          g0[c].x[q] = so == kRegDepth0 ? y[c].x[q] : g0[c].x[q];
+          if (NRS > 1) g1[c].x[q] = so == kRegDepth0 + 1 ? y[c].x[q] : g1[c].x[q];
+          if (NRS > 2) g2[c].x[q] = so == kRegDepth0 + 2 ? y[c].x[q] : g2[c].x[q];
+        }
+    }
     e0 = f0;
     e1 = f1;
   }
@@ -997,10 +1037,14 @@ int launch_post_dna(const PassArgs<R>& a, const int32_t* prog, const int32_t* se
   }
   dim3 grid(a.ntiles, (a.K + W - 1) / W);
   const int4* pg = reinterpret_cast<const int4*>(prog);
+  // register depths: FP32 with at most four categories (the others would
+  // lose occupancy)
+  constexpr int NR = sizeof(R) == 4 ? 3 : 0;
 #define HX_WALK(CMV, EX)                                                                     \
   do {                                                                                       \
-    auto kern = tips == HX_TIPS_CODES ? k_post_walk<R, HX_TIPS_CODES, CMV, EX>               \
-                                      : k_post_walk<R, HX_TIPS_DENSE, CMV, EX>;              \
+    constexpr int NRV = (CMV <= 4 && EX) ? NR : 0;                                          \
+    auto kern = tips == HX_TIPS_CODES ? k_post_walk<R, HX_TIPS_CODES, CMV, EX, NRV>          \
+                                      : k_post_walk<R, HX_TIPS_DENSE, CMV, EX, NRV>;         \
     if (lds > 64 * 1024)                                                                     \
       HX_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),                    \
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
