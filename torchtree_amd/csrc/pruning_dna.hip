@@ -653,7 +653,7 @@ struct PreIn {
 
 template <int TIPS, int CM, bool EXACT, int RS, bool TAU>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
-    !EXACT || (TAU && TIPS == HX_TIPS_CODES) ? 1 : (RS == 0 ? 4 : 3)))) void k_pre_walk(PassArgs<float> a, const float* __restrict__ mats,
+    !EXACT ? 1 : (RS == 0 ? 4 : 3)))) void k_pre_walk(PassArgs<float> a, const float* __restrict__ mats,
                                                   const float* __restrict__ mats_dt,
                                                   const int* __restrict__ prog, int nprog, int W) {
   using R = float;
@@ -981,15 +981,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
 }
 
 // Occupancy and stack: the d loglik / d mats walk with tip codes fits 128
-// VGPRs with every push in memory (4 waves per SIMD); the other variants
-// keep two register stack slots, at 3 waves per SIMD (168 VGPRs) where that
-// needs no spill (a spill's scratch traffic would break the in-order waits).
+// VGPRs with every push in memory (4 waves per SIMD); the d loglik / d tau
+// walk with tip codes keeps one register stack slot at four categories and
+// the others two, at 3 waves per SIMD (168 VGPRs) without spills (a spill's scratch
+// traffic would break the in-order waits).
 template <int CM, bool EXACT>
 static void pre_walk_kernel(int tips, bool tau,
                             void (*&kern)(PassArgs<float>, const float*, const float*, const int*,
                                           int, int)) {
   if (tau)
-    kern = tips == HX_TIPS_CODES ? k_pre_walk<HX_TIPS_CODES, CM, EXACT, 2, true>
+    kern = tips == HX_TIPS_CODES ? k_pre_walk<HX_TIPS_CODES, CM, EXACT, (CM == 4 ? 1 : 2), true>
                                  : k_pre_walk<HX_TIPS_DENSE, CM, EXACT, 2, true>;
   else
     kern = tips == HX_TIPS_CODES ? k_pre_walk<HX_TIPS_CODES, CM, EXACT, 0, false>
