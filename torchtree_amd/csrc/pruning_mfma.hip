@@ -86,6 +86,43 @@ __device__ __forceinline__ R child_val(const PassArgs<R>& a, int ch, int k, int 
   return a.partials[((((size_t)(ch - a.T) * a.K + k) * a.N + n) * a.C + c) * S + s];
 }
 
+// States s0 .. s0 + KS - 1 of child ch at absolute pattern n, category c (0
+// outside): one code load for a tip, 16-byte loads for an internal child's
+// row when S is a multiple of 4 (rows then start 16-byte aligned).
+template <typename R, int TIPS, int KS>
+__device__ __forceinline__ void child_row(const PassArgs<R>& a, int ch, int k, int n, int c, int s0,
+                                          int S, R (&out)[KS]) {
+#pragma unroll
+  for (int q = 0; q < KS; ++q) out[q] = R(0);
+  if (n >= a.N || s0 >= S) return;
+  const R* src;
+  if (ch < a.T) {
+    if constexpr (TIPS == HX_TIPS_CODES) src = a.table + (size_t)a.codes[(size_t)ch * a.N + n] * S + s0;
+    else src = a.dense + ((size_t)ch * a.N + n) * S + s0;
+  } else {
+    src = a.partials + ((((size_t)(ch - a.T) * a.K + k) * a.N + n) * a.C + c) * S + s0;
+  }
+  constexpr int VW = 16 / sizeof(R);  // elements per 16-byte load
+  if (S % VW == 0 && s0 % VW == 0) {
+#pragma unroll
+    for (int q = 0; q < KS; q += VW) {
+      if (s0 + q + VW <= S) {
+        if constexpr (sizeof(R) == 4) {
+          const float4 v = *reinterpret_cast<const float4*>(src + q);
+          out[q] = v.x, out[q + 1] = v.y, out[q + 2] = v.z, out[q + 3] = v.w;
+        } else {
+          const double2 v = *reinterpret_cast<const double2*>(src + q);
+          out[q] = v.x, out[q + 1] = v.y;
+        }
+      }
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < KS; ++q)
+      if (s0 + q < S) out[q] = src[q];
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- post-order
@@ -146,11 +183,12 @@ __global__ __launch_bounds__(512) void k_post_mfma(PassArgs<R> a,
     vec4_t<R> acc[MT];
 #pragma unroll
     for (int t = 0; t < MT; ++t) acc[t] = vec4_t<R>{R(0), R(0), R(0), R(0)};
+    R arow[KS];
+    child_row<R, TIPS, KS>(a, chh, k, nbase + nn, c, g * KS, S, arow);
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
-      const R av = child_val<R, TIPS>(a, chh, k, nbase + nn, c, g * KS + ks, S);
 #pragma unroll
-      for (int t = 0; t < MT; ++t) acc[t] = mma(av, Pf[t][ks], acc[t]);
+      for (int t = 0; t < MT; ++t) acc[t] = mma(arow[ks], Pf[t][ks], acc[t]);
     }
     // lane (nn, g): acc[t][r] = Y[m = 4 g + r][i = 16 t + nn]
     if (h == 1) {
@@ -305,9 +343,11 @@ __global__ __launch_bounds__(512) void k_pre_mfma(PassArgs<R> a,
     vec4_t<R> acc[MT];
 #pragma unroll
     for (int t = 0; t < MT; ++t) acc[t] = vec4_t<R>{R(0), R(0), R(0), R(0)};
+    R arow[KS];
+    child_row<R, TIPS, KS>(a, chh, k, nbase + nn, c, g * KS, S, arow);
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
-      const R av = child_val<R, TIPS>(a, chh, k, nbase + nn, c, g * KS + ks, S);
+      const R av = arow[ks];
 #pragma unroll
       for (int t = 0; t < MT; ++t) {
         R bv;
