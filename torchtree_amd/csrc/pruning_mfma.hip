@@ -295,7 +295,8 @@ __global__ __launch_bounds__(512) void k_post_mfma(PassArgs<R> a,
 
 // ----------------------------------------------------------------- pre-order
 template <typename R, int MT, int TIPS, bool ROOT>
-__global__ __launch_bounds__(512) void k_pre_mfma(PassArgs<R> a,
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
+    sizeof(R) == 4 && MT <= 2 ? 4 : 1))) void k_pre_mfma(PassArgs<R> a,
                                                   const int32_t* __restrict__ nodes, int S) {
   constexpr int SP = 16 * MT, KS = 4 * MT;
   extern __shared__ __align__(16) unsigned char smem_raw[];
