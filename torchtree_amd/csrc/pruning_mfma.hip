@@ -304,6 +304,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
   R* Yx = reinterpret_cast<R*>(smem_raw);  // [2][C][16][SP]: P L, then Ut transposes
   R* dens = Yx + 2 * C * kTile * SP;           // [C][16]
   R* mxx = dens + C * kTile;                   // [2][C][16]
+  R* Lx = mxx + 2 * C * kTile;                 // [2][C][16][SP]: the child's partial tile
 
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int h = wave / C, c = wave - h * C;
@@ -335,6 +336,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
     for (int j = 0; j < MT; ++j) G[i][j] = vec4_t<R>{R(0), R(0), R(0), R(0)};
   R* mine = Yx + (size_t)(h * C + c) * kTile * SP;
   const R* other = Yx + (size_t)((1 - h) * C + c) * kTile * SP;
+  R* lt = Lx + (size_t)(h * C + c) * kTile * SP;  // this wave's L tile [16 patterns][SP]
 
   const int t0 = chunk * a.tiles_per_chunk;
   const int t1 = min(t0 + a.tiles_per_chunk, a.ntiles);
@@ -346,6 +348,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
     for (int t = 0; t < MT; ++t) acc[t] = vec4_t<R>{R(0), R(0), R(0), R(0)};
     R arow[KS];
     child_row<R, TIPS, KS>(a, chh, k, nbase + nn, c, g * KS, S, arow);
+    // the tile of L, read back transposed as the statistic's B operand (this
+    // wave's own rows: no barrier)
+#pragma unroll
+    for (int q = 0; q < KS; ++q) lt[nn * SP + g * KS + q] = arow[q];
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
       const R av = arow[ks];
@@ -407,8 +413,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
       // register u[Mt][ks] scaled by coef[ks]; B = L[4 g + ks][16 Nt + nn]
       R bv[MT];
 #pragma unroll
-      for (int nt = 0; nt < MT; ++nt)
-        bv[nt] = child_val<R, TIPS>(a, chh, k, nbase + acc_row<R>(g, ks), c, 16 * nt + nn, S);
+      for (int nt = 0; nt < MT; ++nt) bv[nt] = lt[acc_row<R>(g, ks) * SP + 16 * nt + nn];
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) {
         const R av = coef[ks] * u[mt][ks];
@@ -416,12 +421,13 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
         for (int nt = 0; nt < MT; ++nt) G[mt][nt] = mma(av, bv[nt], G[mt][nt]);
       }
     }
-    // transpose Ut through this wave's own LDS rows (P L rows are no longer read)
+    // transpose Ut through this wave's own LDS rows: the other child's waves
+    // read these P L rows before the previous barrier, and only this wave
+    // reads the transpose, so no barrier is needed here
 #pragma unroll
     for (int t = 0; t < MT; ++t)
 #pragma unroll
       for (int r = 0; r < 4; ++r) mine[(acc_row<R>(g, r)) * SP + 16 * t + nn] = u[t][r];
-    __syncthreads();
     vec4_t<R> nu[MT];
     if (internal) {
 #pragma unroll
@@ -463,7 +469,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
         }
       }
     }
-    __syncthreads();
+    // no barrier: the next tile's first barrier orders this tile's reads of
+    // dens and mxx before their next writes, and each wave rewrites only its
+    // own rows of Yx and Lx before it
   }
   if (a.mats_dt != nullptr) {
     const R* q = a.mats_dt + (((size_t)k * a.B + chh) * C + c) * S * S;
@@ -503,7 +511,7 @@ template <typename R>
 static size_t mfma_lds(int C, int S, bool pre) {
   const int MT = S <= 16 ? 1 : (S <= 32 ? 2 : 4);
   const size_t SP = 16 * MT, rb = sizeof(R);
-  if (pre) return (2 * C * kTile * SP + 3 * C * kTile) * rb + 64;
+  if (pre) return (4 * C * kTile * SP + 3 * C * kTile) * rb + 64;
   return (C * kTile * SP + C * kTile) * rb + (size_t)(C * kTile + C * SP) * 8 + 3 * kTile * 4 + 64;
 }
 
