@@ -76,9 +76,10 @@ def walk_need(post, T):
     return need[post[-1][0]]
 
 
-def pre_walk_used(S, C, rb):
+def pre_walk_used(S, C, rb, N, K):
     """The four-state FP32 pre-order runs as one walk (mirrors use_pre_walk, csrc/common.hpp)."""
-    return S == 4 and rb == 4 and C <= 4
+    groups = ((N + 63) // 64 + 3) // 4 * K
+    return S == 4 and rb == 4 and C <= 4 and groups >= 256
 
 
 def algorithmic_bytes(T, N, C, K, S, rb, nchunks):
@@ -105,7 +106,7 @@ def algorithmic_bytes(T, N, C, K, S, rb, nchunks):
         post_b = (I - 1) * part + tips + mats + weights
     else:
         post_b = 2 * (I - 1) * part + 2 * (I - 1) * K * N * 4 + tips + mats + weights
-    if pre_walk_used(S, C, rb):
+    if pre_walk_used(S, C, rb, N, K):
         pre_b = (I - 1) * part + K * tips + mats + weights + 2 * gpart
     else:
         pre_b = 3 * (I - 1) * part + tips + mats + weights + gpart
@@ -233,14 +234,14 @@ def main():
         "roofline": {
             "bound": "hbm",
             "kernel": (("k_pre_walk (one launch, whole tree) + k_reduce_g"
-                        if pre_walk_used(S, wl.C, 4) else
+                        if pre_walk_used(S, wl.C, 4, wl.N, wl.K) else
                         f"k_pre ({launches - 1} depth-level launches) + k_reduce_g")
                        if dom == "preorder" else "k_post_walk (one launch, whole tree) + k_reduce_root"),
             "achieved": dom_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dom_gbs / HBM_PEAK_GBS,
             "traffic": measured_traffic(args, dom), "algorithmic_bytes": dom_b, "pass_ms": dom_ms,
             # the walks are one launch for the whole tree (pass time includes the
             # small reduction that follows); level kernels one launch per level
-            "avg_launch_us": dom_ms * 1e3 / (1 if (dom == "postorder" or pre_walk_used(S, wl.C, 4))
+            "avg_launch_us": dom_ms * 1e3 / (1 if (dom == "postorder" or pre_walk_used(S, wl.C, 4, wl.N, wl.K))
                                              else max(1, launches - 1)),
         },
         "other_pass": {"postorder_gbs": post_gbs, "preorder_gbs": pre_gbs,
@@ -393,7 +394,7 @@ def _nchunks(N, C, K, rb=4):
     """Chunk count of the S=4 kernels (mirrors make_tiling in csrc/common.hpp)."""
     ppb = 64
     ntiles = (N + ppb - 1) // ppb
-    if pre_walk_used(4, C, rb):
+    if pre_walk_used(4, C, rb, N, K):
         return (ntiles + 3) // 4  # four tiles per pre-order walk workgroup
     nch = max(1, min(ntiles, 128))
     tpc = (ntiles + nch - 1) // nch

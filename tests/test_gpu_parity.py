@@ -158,7 +158,12 @@ def test_engine_grad_model(gpu, dtype, rtol_l, rtol_g):
         assert err < rtol_g, (name, err)
 
 
-@pytest.mark.parametrize("T,N,C,dense", [(70, 700, 8, False), (40, 260, 16, False), (30, 333, 4, True)])
+# (24, 33000, ., .) with K = 2 has ceil(516 / 4) * 2 = 258 workgroups: the FP32
+# pre-order runs as the walk (C = 1, 3, 4 and dense tips); the smaller shapes
+# run it by level
+@pytest.mark.parametrize("T,N,C,dense", [(70, 700, 8, False), (40, 260, 16, False), (30, 333, 4, True),
+                                         (24, 33000, 4, False), (24, 33000, 1, False),
+                                         (24, 33000, 3, False), (24, 33000, 4, True)])
 @pytest.mark.parametrize("dtype,rtol_l,rtol_g", [(F64, 1e-12, 1e-10), (torch.float32, 1e-6, 1e-5)])
 def test_dropin_dna_shapes(gpu, T, N, C, dense, dtype, rtol_l, rtol_g):
     """Four-state kernels beyond the default shape: 8 and 16 categories (one-level

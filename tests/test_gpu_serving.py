@@ -87,3 +87,35 @@ def test_served_matches_oracle(gpu, dtype, rl, rg):
         g = b.grad[:, :-1]  # slot 2T-3 is a padded (fixed) length
         err = float((req.dbls.cpu()[:, :-1] - g).norm() / g.norm())
         assert err < rg, (sp["name"], err)
+
+
+def test_served_walk_matches_oracle(gpu):
+    """A DNA request large enough for the FP32 pre-order walk (d loglik / d tau
+    variant: 24 taxa, 33,000 patterns, K = 2, 258 workgroups) against the oracle."""
+    print("case: served DNA walk vs oracle", flush=True)
+    from torchtree_amd import serving, synthetic
+
+    rng = np.random.default_rng(77)
+    sp = synthetic.random_job_spec("dna", 24, 33000, rng)
+    post = [tuple(int(x) for x in r) for r in sp["topo"].postorder]
+    jb = serving.ResidentJob("walk", post, sp["codes"], sp["table"], sp["weights"], sp["q_norm"],
+                             sp["freqs"], sp["rates"], sp["props"], dtype=torch.float32,
+                             device=gpu.index or 0, kind="dna")
+    bls = synthetic.draw_bls(sp, 2, 13)
+    req = serving.Request(0, jb, bls)
+    serving.run_sequential([req], device=gpu.index or 0)
+    b = bls.clone().requires_grad_(True)
+    tau = b.unsqueeze(-1) * torch.as_tensor(sp["rates"]).reshape(1, 1, -1)
+    mats = omodels.p_t_eigh(sp["q_norm"], torch.as_tensor(sp["freqs"]), tau)
+    tbl = torch.as_tensor(sp["table"])
+    tips = [tbl[torch.as_tensor(c).long()].t().contiguous() for c in sp["codes"]]
+    ref = opruning.treelik_discrete_rescaled(tips + [None] * (len(tips) - 1),
+                                             torch.as_tensor(sp["weights"]), post, mats,
+                                             torch.as_tensor(sp["freqs"]).reshape(1, -1),
+                                             torch.as_tensor(sp["props"]).reshape(-1, 1, 1)).reshape(-1)
+    ref.sum().backward()
+    rel = float(((req.loglik.cpu() - ref.detach()).abs() / ref.detach().abs()).max())
+    assert rel < 1e-6, rel
+    g = b.grad[:, :-1]  # slot 2T-3 is a padded (fixed) length
+    err = float((req.dbls.cpu()[:, :-1] - g).norm() / g.norm())
+    assert err < 1e-5, err

@@ -236,12 +236,12 @@ static int run_pre(const CallArgs& c, double* out, hipStream_t st) {
   const hx_plan_s* plan = c.plan;
   const int D = (int)plan->pre_off.size() - 1;
   if constexpr (std::is_same<R, float>::value) {
-    if (use_pre_walk(c.S, c.C, 4)) {
+    if (use_pre_walk(c.S, c.C, 4, c.N, c.K)) {
       const int rc = launch_pre_walk(a, plan->d_prew, plan->I, c.tip_kind, st);
       if (rc != HX_OK) return rc;
     }
   }
-  for (int d = 0; d < D && !(sizeof(R) == 4 && use_pre_walk(c.S, c.C, 4)); ++d) {
+  for (int d = 0; d < D && !(sizeof(R) == 4 && use_pre_walk(c.S, c.C, 4, c.N, c.K)); ++d) {
     const int off = plan->pre_off[d], width = plan->pre_off[d + 1] - off;
     const int rc = c.S == 4
                        ? launch_pre_dna<R>(a, plan->d_pre + off, width, c.tip_kind, d == 0, st)

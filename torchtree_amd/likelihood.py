@@ -192,12 +192,13 @@ class Pruning:
             tips_p, table_p = self.tips.dense_as(dtype), None
         return K, B, C, S, code, tips_p, table_p
 
-    def _launches(self, which, S, C, code):
+    def _launches(self, which, S, C, code, K):
         """Kernel launches of one pass (the walks are one launch for the whole tree)."""
         if S == 4 and which == "post":
             return 2 + int(self.tips.kind == _lib.HX_TIPS_CODES)  # pad codes, walk, root sums
-        if S == 4 and which == "pre" and code == _lib.HX_F32 and C <= 4:
-            return 2  # walk, statistic reduction
+        groups = ((self.N + 63) // 64 + 3) // 4 * K
+        if S == 4 and which == "pre" and code == _lib.HX_F32 and C <= 4 and groups >= 256:
+            return 2  # walk, statistic reduction (mirrors use_pre_walk, csrc/common.hpp)
         return (self.plan.post_levels if which == "post" else self.plan.pre_levels) + 1
 
     def forward(self, mats, freqs, props, want_root_grads=True):
@@ -215,7 +216,7 @@ class Pruning:
         out = torch.empty(K, dtype=torch.float64, device=dev)
         dfreqs = torch.empty(K, S, dtype=torch.float64, device=dev) if want_root_grads else None
         dprops = torch.empty(K, C, dtype=torch.float64, device=dev) if want_root_grads else None
-        with timing.phase("postorder", self._launches("post", S, C, code)):
+        with timing.phase("postorder", self._launches("post", S, C, code, K)):
             rc = lib.hx_loglik(
             self.plan.handle, S, self.N, C, K, code, self.tips.kind,
             _lib.ptr(tips_p), _lib.ptr(table_p), self.tips.ncodes,
@@ -234,7 +235,7 @@ class Pruning:
         K, B, C, S, code, tips_p, table_p = self._common(mats, freqs, props, dtype)
         dev = mats.device
         dmats = torch.empty(K, B, C, S, S, dtype=torch.float64, device=dev)
-        with timing.phase("preorder", self._launches("pre", S, C, code)):
+        with timing.phase("preorder", self._launches("pre", S, C, code, K)):
             rc = lib.hx_loglik_grad_mats(
             self.plan.handle, S, self.N, C, K, code, self.tips.kind,
             _lib.ptr(tips_p), _lib.ptr(table_p), self.tips.ncodes,
