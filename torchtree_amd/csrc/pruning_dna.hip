@@ -1,6 +1,6 @@
 // Four-state (nucleotide) pruning kernels: one fused post-order walk of the
-// whole tree per pattern tile, and the pre-order adjoint one tree level per
-// launch.
+// whole tree per pattern tile, and the pre-order adjoint either as one walk
+// of the whole tree or one tree level per launch.
 //
 // Reference: calculate_treelikelihood_discrete* (torchtree/evolution/
 // tree_likelihood.py:40-278) at S = 4, where each node update is two 4x4
@@ -13,9 +13,11 @@
 //
 // Post-order (k_post_walk): lane = pattern with all C categories in the lane,
 // one wave per (tile, sample), walking the tree program; each internal
-// partial is written once and never read back except by the lane that wrote
-// it, when a child waited on the stack (an L2 hit). Pre-order (k_pre): thread
-// = (pattern, category), one block per (chunk, sample, parent) of a level.
+// partial is written once and read back only by the lane that wrote it, when
+// a child waited on the stack at a depth not kept in registers. Pre-order:
+// k_pre_walk (FP32, C <= 4, enough work to fill the chip) walks the tree
+// depth first with the adjoints in registers; otherwise k_pre runs one tree
+// level per launch with thread = (pattern, category).
 #include "pruning.hpp"
 #include "walk.hpp"
 

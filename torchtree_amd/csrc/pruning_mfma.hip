@@ -11,7 +11,8 @@
 //   forward    Y[m][i]  = sum_j L[m][j] P[i][j]        A = L (patterns x states),
 //                                                       B = P^T held in registers
 //   gradient   G[i][j] += sum_m U'[m][i] L[m][j]        contraction over patterns;
-//                                                       A is the wave's own U' registers
+//                                                       A is the wave's own U' registers,
+//                                                       B the child's tile staged in LDS
 //   transpose  U[m][j]  = sum_i Ut[m][i] P[i][j]        A from an LDS transpose of Ut
 // S is padded to SP = 16 * MT states; K index k = 4 ks + g of a step ks maps to
 // state g * (4 MT) + ks (forward / transpose) or to pattern 4 g + ks (gradient),
@@ -69,21 +70,6 @@ __device__ __forceinline__ T row_sum16(T v) {
 #pragma unroll
   for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o);
   return v;
-}
-
-// Child partial of child ch at absolute pattern n, category c, state s (0 outside).
-template <typename R, int TIPS>
-__device__ __forceinline__ R child_val(const PassArgs<R>& a, int ch, int k, int n, int c, int s,
-                                       int S) {
-  if (n >= a.N || s >= S) return R(0);
-  if (ch < a.T) {
-    if constexpr (TIPS == HX_TIPS_CODES) {
-      return a.table[(size_t)a.codes[(size_t)ch * a.N + n] * S + s];
-    } else {
-      return a.dense[((size_t)ch * a.N + n) * S + s];
-    }
-  }
-  return a.partials[((((size_t)(ch - a.T) * a.K + k) * a.N + n) * a.C + c) * S + s];
 }
 
 // States s0 .. s0 + KS - 1 of child ch at absolute pattern n, category c (0

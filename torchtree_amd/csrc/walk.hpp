@@ -120,9 +120,6 @@ inline bool build_walk(int T, const std::vector<int32_t>& kids, const int32_t* p
          walk_need <= 255 && walk_need == need[root] && walk[5] == 0;
 }
 
-}  // namespace hx
-
-namespace hx {
 
 // Pre-order walk program: one entry per internal node, depth first from the
 // root. An entry processes parent p with children a and b given p's adjoint U
