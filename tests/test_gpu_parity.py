@@ -201,3 +201,63 @@ def test_dropin_dna_shapes(gpu, T, N, C, dense, dtype, rtol_l, rtol_g):
         a = a.double().cpu()
         err = float((a - b).norm() / b.norm())
         assert err < rtol_g, err
+
+
+def _shape_post(shape, T):
+    """Post-order triples of a caterpillar or a balanced rooted tree."""
+    post, nxt = [], T
+    if shape == "caterpillar":
+        prev = 0
+        for t in range(1, T):
+            post.append((nxt, prev, t))
+            prev = nxt
+            nxt += 1
+        return post
+    level = list(range(T))
+    while len(level) > 1:
+        new = []
+        for i in range(0, len(level) - 1, 2):
+            post.append((nxt, level[i], level[i + 1]))
+            new.append(nxt)
+            nxt += 1
+        if len(level) % 2:
+            new.append(level[-1])
+        level = new
+    return post
+
+
+@pytest.mark.parametrize("shape", ["caterpillar", "balanced"])
+def test_pre_walk_tree_shapes(gpu, shape):
+    """The FP32 pre-order walk on trees with no pushes (caterpillar) and with
+    the deepest stack (balanced, 32 taxa: depth 4, every push through memory)
+    at a walk-sized pattern count, against the FP64 oracle."""
+    _flush(f"case: pre-order walk {shape}")
+    from torchtree_amd import likelihood as hl
+
+    T, N, K, C = 32, 33000, 2, 4
+    rng = np.random.default_rng(9)
+    post = _shape_post(shape, T)
+    codes = rng.integers(0, 4, size=(T, N))
+    tbl = torch.eye(4, dtype=F64)
+    tips = [tbl[torch.as_tensor(c)].t().contiguous() for c in codes]
+    weights = torch.as_tensor(rng.integers(1, 4, size=N), dtype=F64)
+    freqs0 = torch.as_tensor(rng.dirichlet(np.ones(4) * 5), dtype=F64)
+    q = omodels.gtr_q(torch.as_tensor(rng.gamma(2.0, 0.5, size=6), dtype=F64), freqs0)
+    rates = torch.linspace(0.2, 2.5, C, dtype=F64)
+    bls = torch.as_tensor(rng.exponential(0.1, size=(K, 2 * T - 2)), dtype=F64)
+    tau = bls.unsqueeze(-1) * rates.reshape(1, 1, -1)
+    mats = omodels.p_t_eigh(q, freqs0, tau).detach().requires_grad_(True)
+    freqs = freqs0.reshape(1, -1).clone().requires_grad_(True)
+    pr = torch.full((C, 1, 1), 1.0 / C, dtype=F64).requires_grad_(True)
+    ref = opruning.treelik_discrete_rescaled(tips + [None] * (T - 1), weights, post, mats, freqs, pr)
+    ref.sum().backward()
+    m2 = mats.detach().to(gpu, torch.float32).requires_grad_(True)
+    f2 = freqs.detach().to(gpu, torch.float32).requires_grad_(True)
+    p2 = pr.detach().to(gpu, torch.float32).requires_grad_(True)
+    got = hl.calculate_treelikelihood_discrete(tips + [None] * (T - 1), weights, post, m2, f2, p2)
+    rel = (got.double().cpu() - ref.detach()).abs() / ref.detach().abs()
+    assert float(rel.max()) < 1e-6
+    got.sum().backward()
+    for a, b in ((m2.grad, mats.grad), (f2.grad, freqs.grad), (p2.grad, pr.grad)):
+        err = float((a.double().cpu() - b).norm() / b.norm())
+        assert err < 1e-5, err
