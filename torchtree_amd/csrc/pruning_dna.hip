@@ -369,7 +369,10 @@ __global__ __launch_bounds__(CM >= 8 ? 256 : 1024) void k_post_walk(PassArgs<R> 
     for (int c = 0; c < CM; ++c)
       if (EXACT || c < C) st4(p + (size_t)c * cstride, y[c]);
     est[so * 64 + lane] = E;
-    if (NRS > 0) {
+    // only a result that will wait on the stack (the next entry does not
+    // consume it) needs its register copy
+    const bool waits = (f1.z & 0xff) != 1 && ((f1.z >> 8) & 0xff) != 1;
+    if (NRS > 0 && waits && in_regs(so)) {
 #pragma unroll
       for (int c = 0; c < CM; ++c)
 #pragma unroll
