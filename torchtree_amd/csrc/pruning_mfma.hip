@@ -15,9 +15,11 @@
 //                                                       B the child's tile staged in LDS
 //   transpose  U[m][j]  = sum_i Ut[m][i] P[i][j]        A from an LDS transpose of Ut
 // S is padded to SP = 16 * MT states; K index k = 4 ks + g of a step ks maps to
-// state g * (4 MT) + ks (forward / transpose) or to pattern 4 g + ks (gradient),
+// state g * KV + ks (forward / transpose) or to pattern 4 g + ks (gradient),
 // so each lane's A operand is a contiguous run and the accumulator layout of
-// one product is the operand layout of the next. The 16x16x4 f32 layout used:
+// one product is the operand layout of the next. The contractions over states
+// take KV steps with 4 KV >= S: KV = 4 MT in general and 5 for 17..20 states
+// (amino acids), which skips the padded states' steps. The 16x16x4 f32 layout used:
 // A[i][k] from lane i + 16 k, B[k][j] from lane j + 16 k, D[i][j] in lane
 // j + 16 (i / 4), register i % 4 for FP32 and lane j + 16 (i % 4), register
 // i / 4 for FP64 (acc_row; checked on the device by hx_selftest_mfma_layout).
@@ -74,7 +76,8 @@ __device__ __forceinline__ T row_sum16(T v) {
 
 // States s0 .. s0 + KS - 1 of child ch at absolute pattern n, category c (0
 // outside): one code load for a tip, 16-byte loads for an internal child's
-// row when S is a multiple of 4 (rows then start 16-byte aligned).
+// row when S, s0 and KS are multiples of the vector width (rows then start
+// 16-byte aligned and each load fills whole elements of out).
 template <typename R, int TIPS, int KS>
 __device__ __forceinline__ void child_row(const PassArgs<R>& a, int ch, int k, int n, int c, int s0,
                                           int S, R (&out)[KS]) {
@@ -89,7 +92,7 @@ __device__ __forceinline__ void child_row(const PassArgs<R>& a, int ch, int k, i
     src = a.partials + ((((size_t)(ch - a.T) * a.K + k) * a.N + n) * a.C + c) * S + s0;
   }
   constexpr int VW = 16 / sizeof(R);  // elements per 16-byte load
-  if (S % VW == 0 && s0 % VW == 0) {
+  if (KS % VW == 0 && S % VW == 0 && s0 % VW == 0) {
 #pragma unroll
     for (int q = 0; q < KS; q += VW) {
       if (s0 + q + VW <= S) {
@@ -112,10 +115,10 @@ __device__ __forceinline__ void child_row(const PassArgs<R>& a, int ch, int k, i
 }  // namespace
 
 // ---------------------------------------------------------------- post-order
-template <typename R, int MT, int TIPS, bool ROOT>
+template <typename R, int MT, int KV, int TIPS, bool ROOT>
 __global__ __launch_bounds__(512) void k_post_mfma(PassArgs<R> a,
                                                    const int32_t* __restrict__ nodes, int S) {
-  constexpr int SP = 16 * MT, KS = 4 * MT;
+  constexpr int SP = 16 * MT;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int C = a.C;
   R* Yx = reinterpret_cast<R*>(smem_raw);       // [C][16][SP] child-1 products
@@ -132,14 +135,14 @@ __global__ __launch_bounds__(512) void k_post_mfma(PassArgs<R> a,
   const int ch0 = a.kids[2 * v], ch1 = a.kids[2 * v + 1];
   const int chh = h ? ch1 : ch0;
 
-  R Pf[MT][KS];  // Pf[t][ks] = P[16 t + nn][g KS + ks]
+  R Pf[MT][KV];  // Pf[t][ks] = P[16 t + nn][g KV + ks]
   {
     const R* pm = a.mats + (((size_t)k * a.B + chh) * C + c) * S * S;
 #pragma unroll
     for (int t = 0; t < MT; ++t)
 #pragma unroll
-      for (int ks = 0; ks < KS; ++ks) {
-        const int i = 16 * t + nn, j = g * KS + ks;
+      for (int ks = 0; ks < KV; ++ks) {
+        const int i = 16 * t + nn, j = g * KV + ks;
         Pf[t][ks] = (i < S && j < S) ? pm[i * S + j] : R(0);
       }
   }
@@ -169,10 +172,10 @@ __global__ __launch_bounds__(512) void k_post_mfma(PassArgs<R> a,
     vec4_t<R> acc[MT];
 #pragma unroll
     for (int t = 0; t < MT; ++t) acc[t] = vec4_t<R>{R(0), R(0), R(0), R(0)};
-    R arow[KS];
-    child_row<R, TIPS, KS>(a, chh, k, nbase + nn, c, g * KS, S, arow);
+    R arow[KV];
+    child_row<R, TIPS, KV>(a, chh, k, nbase + nn, c, g * KV, S, arow);
 #pragma unroll
-    for (int ks = 0; ks < KS; ++ks) {
+    for (int ks = 0; ks < KV; ++ks) {
 #pragma unroll
       for (int t = 0; t < MT; ++t) acc[t] = mma(arow[ks], Pf[t][ks], acc[t]);
     }
@@ -280,11 +283,11 @@ __global__ __launch_bounds__(512) void k_post_mfma(PassArgs<R> a,
 }
 
 // ----------------------------------------------------------------- pre-order
-template <typename R, int MT, int TIPS, bool ROOT>
+template <typename R, int MT, int KV, int TIPS, bool ROOT>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
     sizeof(R) == 4 && MT <= 2 ? 4 : 1))) void k_pre_mfma(PassArgs<R> a,
                                                   const int32_t* __restrict__ nodes, int S) {
-  constexpr int SP = 16 * MT, KS = 4 * MT;
+  constexpr int SP = 16 * MT;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int C = a.C;
   R* Yx = reinterpret_cast<R*>(smem_raw);  // [2][C][16][SP]: P L, then Ut transposes
@@ -301,16 +304,16 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
   const bool internal = chh >= a.T;
   const R* pm = a.mats + (((size_t)k * a.B + chh) * C + c) * S * S;
 
-  // Pf[t][ks] = P[16 t + nn][g KS + ks]; at 64 padded states the gradient
+  // Pf[t][ks] = P[16 t + nn][g KV + ks]; at 64 padded states the gradient
   // accumulators take those registers and P is re-read from L1/L2 per tile
   constexpr bool kPfRegs = MT < 4;
-  R Pf[kPfRegs ? MT : 1][kPfRegs ? KS : 1];
+  R Pf[kPfRegs ? MT : 1][kPfRegs ? KV : 1];
   if constexpr (kPfRegs) {
 #pragma unroll
     for (int t = 0; t < MT; ++t)
 #pragma unroll
-      for (int ks = 0; ks < KS; ++ks) {
-        const int i = 16 * t + nn, j = g * KS + ks;
+      for (int ks = 0; ks < KV; ++ks) {
+        const int i = 16 * t + nn, j = g * KV + ks;
         Pf[t][ks] = (i < S && j < S) ? pm[i * S + j] : R(0);
       }
   }
@@ -323,23 +326,47 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
   R* mine = Yx + (size_t)(h * C + c) * kTile * SP;
   const R* other = Yx + (size_t)((1 - h) * C + c) * kTile * SP;
   R* lt = Lx + (size_t)(h * C + c) * kTile * SP;  // this wave's L tile [16 patterns][SP]
+  if (4 * KV < SP) {
+    // states 4 KV .. SP - 1 of the tile are never written: zero them once
+#pragma unroll
+    for (int q = lane; q < kTile * SP; q += 64) lt[q] = R(0);
+  }
 
   const int t0 = chunk * a.tiles_per_chunk;
   const int t1 = min(t0 + a.tiles_per_chunk, a.ntiles);
   for (int tile = t0; tile < t1; ++tile) {
     const int nbase = tile * kTile;
     // forward P_h L_h
+    // the parent's adjoint for this tile, issued with the child rows so the
+    // two loads overlap (rows m = acc_row(g, r), states 16 t + nn)
+    R up[MT][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int n = nbase + acc_row<R>(g, r);
+#pragma unroll
+      for (int t = 0; t < MT; ++t) {
+        const int i = 16 * t + nn;
+        up[t][r] = R(0);
+        if (n < a.N && i < S) {
+          if constexpr (ROOT) {
+            up[t][r] = a.freqs[(size_t)k * a.freqs_ks + i];
+          } else {
+            up[t][r] = a.partials[((((size_t)(p - a.T) * a.K + k) * a.N + n) * C + c) * S + i];
+          }
+        }
+      }
+    }
     vec4_t<R> acc[MT];
 #pragma unroll
     for (int t = 0; t < MT; ++t) acc[t] = vec4_t<R>{R(0), R(0), R(0), R(0)};
-    R arow[KS];
-    child_row<R, TIPS, KS>(a, chh, k, nbase + nn, c, g * KS, S, arow);
+    R arow[KV];
+    child_row<R, TIPS, KV>(a, chh, k, nbase + nn, c, g * KV, S, arow);
     // the tile of L, read back transposed as the statistic's B operand (this
     // wave's own rows: no barrier)
 #pragma unroll
-    for (int q = 0; q < KS; ++q) lt[nn * SP + g * KS + q] = arow[q];
+    for (int q = 0; q < KV; ++q) lt[nn * SP + g * KV + q] = arow[q];
 #pragma unroll
-    for (int ks = 0; ks < KS; ++ks) {
+    for (int ks = 0; ks < KV; ++ks) {
       const R av = arow[ks];
 #pragma unroll
       for (int t = 0; t < MT; ++t) {
@@ -347,7 +374,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
         if constexpr (kPfRegs) {
           bv = Pf[t][ks];
         } else {
-          const int i = 16 * t + nn, j = g * KS + ks;
+          const int i = 16 * t + nn, j = g * KV + ks;
           bv = (i < S && j < S) ? pm[i * S + j] : R(0);
         }
         acc[t] = mma(av, bv, acc[t]);
@@ -362,20 +389,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
     R u[MT][4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int m = acc_row<R>(g, r), n = nbase + m;
+      const int m = acc_row<R>(g, r);
       R d = R(0);
 #pragma unroll
       for (int t = 0; t < MT; ++t) {
         const int i = 16 * t + nn;
-        R up = R(0);
-        if (n < a.N && i < S) {
-          if constexpr (ROOT) {
-            up = a.freqs[(size_t)k * a.freqs_ks + i];
-          } else {
-            up = a.partials[((((size_t)(p - a.T) * a.K + k) * a.N + n) * C + c) * S + i];
-          }
-        }
-        u[t][r] = up * other[m * SP + i];
+        u[t][r] = up[t][r] * other[m * SP + i];
         d = fma(u[t][r], acc[t][r], d);
       }
       if (h == 0) {
@@ -419,8 +438,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
       for (int t = 0; t < MT; ++t) nu[t] = vec4_t<R>{R(0), R(0), R(0), R(0)};
 #pragma unroll
-      for (int ks = 0; ks < KS; ++ks) {
-        const int i = g * KS + ks;  // K index (ks, g) = state g KS + ks
+      for (int ks = 0; ks < KV; ++ks) {
+        const int i = g * KV + ks;  // K index (ks, g) = state g KV + ks
         const R av = mine[nn * SP + i];
 #pragma unroll
         for (int t = 0; t < MT; ++t) {
@@ -509,25 +528,36 @@ int launch_mfma(bool post, const PassArgs<R>& a, const int32_t* nodes, int width
   const size_t lds = mfma_lds<R>(a.C, S, !post);
   HX_REQUIRE(lds <= 160 * 1024, HX_ERR_UNSUPPORTED, "dense-state kernel LDS exceeds 160 KiB");
   dim3 grid(a.nchunks, a.K, width);
-#define HX_MF(KER, MTV, TV, RV)                                                              \
+#define HX_MF(KER, MTV, KVV, TV, RV)                                                         \
   do {                                                                                       \
-    auto kern = KER<R, MTV, TV, RV>;                                                         \
+    auto kern = KER<R, MTV, KVV, TV, RV>;                                                    \
     if (lds > 64 * 1024)                                                                     \
       HX_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),                    \
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
     kern<<<grid, threads, lds, st>>>(a, nodes, S);                                           \
   } while (0)
-#define HX_MF_ALL(KER, MTV)                                                     \
-  if (tips == HX_TIPS_CODES) {                                                  \
-    if (root) HX_MF(KER, MTV, HX_TIPS_CODES, true); else HX_MF(KER, MTV, HX_TIPS_CODES, false); \
-  } else {                                                                      \
-    if (root) HX_MF(KER, MTV, HX_TIPS_DENSE, true); else HX_MF(KER, MTV, HX_TIPS_DENSE, false); \
+#define HX_MF_ALL(KER, MTV, KVV)                                                             \
+  if (tips == HX_TIPS_CODES) {                                                               \
+    if (root) HX_MF(KER, MTV, KVV, HX_TIPS_CODES, true);                                     \
+    else HX_MF(KER, MTV, KVV, HX_TIPS_CODES, false);                                         \
+  } else {                                                                                   \
+    if (root) HX_MF(KER, MTV, KVV, HX_TIPS_DENSE, true);                                     \
+    else HX_MF(KER, MTV, KVV, HX_TIPS_DENSE, false);                                         \
+  }
+#define HX_MF_KER(KER)                                                                       \
+  if (MT == 1) {                                                                             \
+    HX_MF_ALL(KER, 1, 4)                                                                     \
+  } else if (MT == 2) {                                                                      \
+    if (S <= 20) { HX_MF_ALL(KER, 2, 5) } else { HX_MF_ALL(KER, 2, 8) }                      \
+  } else {                                                                                   \
+    HX_MF_ALL(KER, 4, 16)                                                                    \
   }
   if (post) {
-    if (MT == 1) { HX_MF_ALL(k_post_mfma, 1) } else if (MT == 2) { HX_MF_ALL(k_post_mfma, 2) } else { HX_MF_ALL(k_post_mfma, 4) }
+    HX_MF_KER(k_post_mfma)
   } else {
-    if (MT == 1) { HX_MF_ALL(k_pre_mfma, 1) } else if (MT == 2) { HX_MF_ALL(k_pre_mfma, 2) } else { HX_MF_ALL(k_pre_mfma, 4) }
+    HX_MF_KER(k_pre_mfma)
   }
+#undef HX_MF_KER
 #undef HX_MF_ALL
 #undef HX_MF
   HX_HIP_TRY(hipGetLastError());
