@@ -38,12 +38,21 @@ namespace hx {
 // ------------------------------------------------------------ chunk reductions
 // One block per (sample, quantity): thread t sums parts t, t+256, ... in order,
 // then a fixed tree over the threads; the result does not depend on K.
+// Block (0, 0) also zeroes two runs of words that the dense-state pre-order
+// reads past the end of a slab and must find finite: the padding after the
+// partials, and the head of the root's first slab (the post-order never
+// stores the root's partials).
 __global__ __launch_bounds__(256) void k_reduce_root(const double* __restrict__ rootsums, int K,
                                                      int nparts, int S, int C, double* out_l,
-                                                     double* out_f, double* out_p) {
+                                                     double* out_f, double* out_p, uint32_t* zero0,
+                                                     int nzero0, uint32_t* zero1, int nzero1) {
   __shared__ double red[256];
   const int Q = 1 + S + C;
   const int k = blockIdx.x, q = blockIdx.y;
+  if (k == 0 && q == 0) {
+    for (int i = threadIdx.x; i < nzero0; i += 256) zero0[i] = 0u;
+    for (int i = threadIdx.x; i < nzero1; i += 256) zero1[i] = 0u;
+  }
   double t = 0.0;
   for (int ch = threadIdx.x; ch < nparts; ch += 256) t += rootsums[((size_t)k * nparts + ch) * Q + q];
   red[threadIdx.x] = t;
@@ -93,7 +102,7 @@ __global__ void k_reduce_tau(const double* __restrict__ gtau, double* __restrict
 
 // ------------------------------------------------------------------- host side
 struct WsLayout {
-  size_t partials, scales, rootsums, gpart, total;
+  size_t partials, tail, scales, rootsums, gpart, total;  // tail: padding after the partials
 };
 
 // grad_mode: 0 loglik only, 1 d loglik / d mats, 2 d loglik / d tau.
@@ -106,7 +115,11 @@ static WsLayout ws_layout(const hx_plan_s* plan, int S, int N, int C, int K, int
   size_t off = 0;
   w.partials = off;
   const size_t np = S == 4 ? (size_t)t.ntiles * 64 : (size_t)N;
-  off = align_up(off + I * K * np * C * S * rb, 256);
+  // dense states: the pre-order reads up to 16 C S + 64 elements past the
+  // last (node, sample) slab (rows of the last tile past N, padded states)
+  w.tail = off + I * K * np * C * S * rb;
+  const size_t tail = S == 4 ? 0 : (16 * (size_t)C * S + 64) * rb;
+  off = align_up(w.tail + tail, 256);
   w.scales = off;  // S = 4: 64-byte-aligned copy of the tip codes [T][ntiles * 64]
   if (S == 4) off = align_up(off + (size_t)plan->T * t.ntiles * 64, 256);
   else off = align_up(off + I * K * (size_t)N * 4, 256);
@@ -224,8 +237,16 @@ static int run_post(const CallArgs& c, double* out_l, double* out_f, double* out
       if (rc != HX_OK) return rc;
     }
   }
-  k_reduce_root<<<dim3(c.K, 1 + c.S + c.C), 256, 0, st>>>(a.rootsums, c.K, nparts, c.S, c.C, out_l,
-                                                         out_f, out_p);
+  // dense states: the pre-order reads up to 16 C S + 64 elements past a slab
+  const WsLayout w = ws_layout(c.plan, c.S, c.N, c.C, c.K, c.dtype, c.grad_mode);
+  const size_t reach = c.S == 4 ? 0 : 16 * (size_t)c.C * c.S + 64;
+  const size_t root_slabs = (size_t)c.K * c.N * c.C * c.S;
+  uint32_t* tail = reinterpret_cast<uint32_t*>(static_cast<char*>(c.ws) + w.tail);
+  uint32_t* head = reinterpret_cast<uint32_t*>(a.partials + (size_t)(plan->root - plan->T) * root_slabs);
+  const int nwords = (int)(sizeof(R) / 4);
+  k_reduce_root<<<dim3(c.K, 1 + c.S + c.C), 256, 0, st>>>(
+      a.rootsums, c.K, nparts, c.S, c.C, out_l, out_f, out_p, tail, (int)reach * nwords, head,
+      (int)(reach < root_slabs ? reach : root_slabs) * nwords);
   HX_HIP_TRY(hipGetLastError());
   return HX_OK;
 }

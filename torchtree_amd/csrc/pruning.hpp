@@ -32,6 +32,16 @@ struct PassArgs {
 
 constexpr double kLn2 = 0.69314718055994530942;
 
+// Workgroup barrier that orders LDS only. __syncthreads() also waits for every
+// outstanding global load and store of the wave (vmcnt(0)); the pruning
+// kernels' barriers guard LDS tiles alone, since no wave reads global data
+// that another wave of the same launch writes.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Dense-state (S != 4) launchers, defined in pruning_wide.hip.
 int wide_nt(int C, int S, int rb);
 template <typename R>

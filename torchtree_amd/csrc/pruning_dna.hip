@@ -641,12 +641,6 @@ __global__ __launch_bounds__(64 * CP) void k_pre(PassArgs<R> a, const R* __restr
 // post-order partial has been read) deeper down.
 constexpr int kXS = 68;  // row stride (floats) of the LDS pattern transposes
 
-__device__ __forceinline__ void lds_barrier() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-  __builtin_amdgcn_s_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
 template <int CM>
 struct PreIn {
   V4<float> L[2][CM];

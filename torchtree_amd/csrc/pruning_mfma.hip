@@ -74,6 +74,41 @@ __device__ __forceinline__ T row_sum16(T v) {
   return v;
 }
 
+// The same 16-lane reductions for FP32 through DPP row permutes (no LDS
+// round trip): quad_perm [1,0,3,2] and [2,3,0,1] reduce each quad, the
+// half-row mirror pairs quads, the row mirror pairs half-rows, so every lane
+// ends with its row's result. Every lane of the wave must be active.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float v) {
+  const int x = __builtin_bit_cast(int, v);
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(x, x, CTRL, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float row_max16_dpp(float v) {
+  v = fmaxf(v, dpp_f32<0xB1>(v));
+  v = fmaxf(v, dpp_f32<0x4E>(v));
+  v = fmaxf(v, dpp_f32<0x141>(v));
+  return fmaxf(v, dpp_f32<0x140>(v));
+}
+__device__ __forceinline__ float row_sum16_dpp(float v) {
+  v += dpp_f32<0xB1>(v);
+  v += dpp_f32<0x4E>(v);
+  v += dpp_f32<0x141>(v);
+  return v + dpp_f32<0x140>(v);
+}
+template <typename R>
+__device__ __forceinline__ R rmax16(R v) {
+  if constexpr (sizeof(R) == 4) return row_max16_dpp(v);
+  else return row_max16(v);
+}
+template <typename R>
+__device__ __forceinline__ R rsum16(R v) {
+  if constexpr (sizeof(R) == 4) return row_sum16_dpp(v);
+  else return row_sum16(v);
+}
+// w / den: the hardware reciprocal (1 ulp) in FP32, a division in FP64
+__device__ __forceinline__ float div_(float w, float den) { return w * __builtin_amdgcn_rcpf(den); }
+__device__ __forceinline__ double div_(double w, double den) { return w / den; }
+
 // States s0 .. s0 + KS - 1 of child ch at absolute pattern n, category c (0
 // outside): one code load for a tip, 16-byte loads for an internal child's
 // row when S, s0 and KS are multiples of the vector width (rows then start
@@ -186,7 +221,7 @@ __global__ __launch_bounds__(512) void k_post_mfma(PassArgs<R> a,
 #pragma unroll
         for (int r = 0; r < 4; ++r) Yx[(c * kTile + acc_row<R>(g, r)) * SP + 16 * t + nn] = acc[t][r];
     }
-    __syncthreads();
+    lds_barrier();
     R x[MT][4];
     if (h == 0) {
 #pragma unroll
@@ -201,7 +236,7 @@ __global__ __launch_bounds__(512) void k_post_mfma(PassArgs<R> a,
         if (nn == 0) mx[c * kTile + acc_row<R>(g, r)] = m_;
       }
     }
-    __syncthreads();
+    lds_barrier();
     if (h == 0) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -234,7 +269,7 @@ __global__ __launch_bounds__(512) void k_post_mfma(PassArgs<R> a,
       }
     }
     if constexpr (ROOT) {
-      __syncthreads();
+      lds_barrier();
       if (h == 0) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -252,7 +287,7 @@ __global__ __launch_bounds__(512) void k_post_mfma(PassArgs<R> a,
         }
       }
     }
-    __syncthreads();
+    lds_barrier();
   }
   if constexpr (ROOT) {
     if (h == 0) {
@@ -268,7 +303,7 @@ __global__ __launch_bounds__(512) void k_post_mfma(PassArgs<R> a,
       acc_l += __shfl_xor(acc_l, 16);
       acc_l += __shfl_xor(acc_l, 32);
     }
-    __syncthreads();
+    lds_barrier();
     double* out = a.rootsums + ((size_t)k * a.nchunks + chunk) * (1 + S + C);
     if (h == 0 && c == 0) {
       for (int s = lane; s < S; s += 64) {
@@ -508,15 +543,279 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
   }
 }
 
+// Pre-order for at most 32 padded states with both P operands in registers
+// (the forward's P^T and the transpose's P) for the whole chunk. Each tile
+// issues the same global loads on every path, so the compiler's waits are
+// exact: rows past N and states past S of the workspace partials read the
+// padding that follows them, reads of caller buffers (codes, tables, dense
+// tips, weights) use clamped indices, and every such value is then zeroed.
+// Addresses are a (node, sample) slab base plus 32-bit element offsets,
+// (n C + c) S + s. A tip child's codes for the chunk are staged in LDS once.
+// Same products and barriers as k_pre_mfma.
+template <typename R, int MT, int KV, int TIPS, bool ROOT>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
+    sizeof(R) == 4 ? 4 : 1))) void k_pre_mfma_r(PassArgs<R> a, const int32_t* __restrict__ nodes,
+                                                int S) {
+  static_assert(MT <= 2, "P operands in registers need at most 32 padded states");
+  constexpr int SP = 16 * MT;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int C = a.C;
+  R* Yx = reinterpret_cast<R*>(smem_raw);  // [2][C][16][SP]: P L, then Ut transposes
+  R* dens = Yx + 2 * C * kTile * SP;       // [C][16]
+  R* mxx = dens + C * kTile;               // [2][C][16]
+  R* Lx = mxx + 2 * C * kTile;             // [2][C][16][SP]: the child's partial tile
+  uint8_t* ctile = reinterpret_cast<uint8_t*>(Lx + 2 * C * kTile * SP);  // [2][tpc * 16] codes
+
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int h = wave / C, c = wave - h * C;
+  const int nn = lane & 15, g = lane >> 4;
+  const int chunk = blockIdx.x, k = blockIdx.y, p = nodes[blockIdx.z];
+  const int ch0 = a.kids[2 * p], ch1 = a.kids[2 * p + 1];
+  const int chh = h ? ch1 : ch0;
+  const bool internal = chh >= a.T;
+  const int CS = C * S;
+  const size_t slab = (size_t)a.N * CS;
+  const R* pslab = a.partials + ((size_t)(p - a.T) * a.K + k) * slab;
+  R* cslab = a.partials + ((size_t)(internal ? chh - a.T : 0) * a.K + k) * slab;
+  const R* pm = a.mats + (((size_t)k * a.B + chh) * C + c) * S * S;
+  const int t0 = chunk * a.tiles_per_chunk;
+  const int t1 = min(t0 + a.tiles_per_chunk, a.ntiles);
+  const int npat = a.tiles_per_chunk * kTile;
+
+  if constexpr (TIPS == HX_TIPS_CODES) {
+    // the chunk's codes of each tip child (0 past N)
+    for (int q = threadIdx.x; q < 2 * npat; q += blockDim.x) {
+      const int hh = q >= npat, m = q - hh * npat, n = t0 * kTile + m;
+      const int chq = hh ? ch1 : ch0;
+      ctile[q] = (chq < a.T && n < a.N) ? a.codes[(size_t)chq * a.N + n] : 0;
+    }
+  }
+
+  // Pf[t][ks] = P[16 t + nn][g KV + ks] (forward), Pt[t][ks] = P[g KV + ks][16 t + nn]
+  // (transpose)
+  R Pf[MT][KV], Pt[MT][KV];
+#pragma unroll
+  for (int t = 0; t < MT; ++t)
+#pragma unroll
+    for (int ks = 0; ks < KV; ++ks) {
+      const int i = 16 * t + nn, j = g * KV + ks;
+      const bool ok = i < S && j < S;
+      Pf[t][ks] = ok ? pm[i * S + j] : R(0);
+      Pt[t][ks] = ok ? pm[j * S + i] : R(0);
+    }
+  const R pc = a.props[(size_t)k * a.props_ks + c];
+  vec4_t<R> G[MT][MT];  // G[Mt][Nt] lane (nn, g) reg r = dlogL/dP[16 Mt + acc_row(g, r)][16 Nt + nn]
+#pragma unroll
+  for (int i = 0; i < MT; ++i)
+#pragma unroll
+    for (int j = 0; j < MT; ++j) G[i][j] = vec4_t<R>{R(0), R(0), R(0), R(0)};
+  R* mine = Yx + (size_t)(h * C + c) * kTile * SP;
+  const R* other = Yx + (size_t)((1 - h) * C + c) * kTile * SP;
+  R* lt = Lx + (size_t)(h * C + c) * kTile * SP;  // this wave's L tile [16 patterns][SP]
+  if (4 * KV < SP) {
+    // states 4 KV .. SP - 1 of the tile are never written
+    for (int q = lane; q < kTile * SP; q += 64) lt[q] = R(0);
+  }
+  bool stok[MT];
+#pragma unroll
+  for (int t = 0; t < MT; ++t) stok[t] = 16 * t + nn < S;
+  R upr[MT];  // root: the frequencies
+  if constexpr (ROOT) {
+#pragma unroll
+    for (int t = 0; t < MT; ++t) {
+      const R v = a.freqs[(size_t)k * a.freqs_ks + (stok[t] ? 16 * t + nn : 0)];
+      upr[t] = stok[t] ? v : R(0);
+    }
+  }
+  // lane offsets within a slab: rows acc_row(g, r) = acc_row(g, 0) + r (FP32) or + 4 r (FP64)
+  const uint32_t rstep = sizeof(R) == 4 ? CS : 4 * CS;
+  const uint32_t ub = (uint32_t)(acc_row<R>(g, 0) * CS + c * S + nn);
+  const uint32_t cb = (uint32_t)(nn * CS + c * S + g * KV);
+  const uint8_t* mycodes = ctile + h * npat;
+  lds_barrier();
+
+  for (int tile = t0; tile < t1; ++tile) {
+    const int nbase = tile * kTile;
+    const uint32_t tb = (uint32_t)nbase * (uint32_t)CS;
+    // child rows first (the forward waits for them alone), then the parent's
+    // adjoint, then the weights
+    R arow[KV];
+    {
+      const bool nok = nbase + nn < a.N;
+      const R* bp;
+      uint32_t base, lim;
+      if (internal) {
+        bp = cslab, base = tb + cb, lim = 0xffffffffu;
+      } else if constexpr (TIPS == HX_TIPS_CODES) {
+        const int code = mycodes[(tile - t0) * kTile + nn];
+        bp = a.table, base = (uint32_t)(code * S + g * KV), lim = (uint32_t)(a.ncodes * S - 1);
+      } else {
+        bp = a.dense + (size_t)chh * a.N * S;
+        base = (uint32_t)(min(nbase + nn, a.N - 1) * S + g * KV), lim = (uint32_t)(a.N * S - 1);
+      }
+#pragma unroll
+      for (int q = 0; q < KV; ++q) arow[q] = bp[min(base + q, lim)] * ((nok && g * KV + q < S) ? R(1) : R(0));
+    }
+    bool rok[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) rok[r] = nbase + acc_row<R>(g, r) < a.N;
+    R up[MT][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int t = 0; t < MT; ++t) {
+        if constexpr (ROOT) {
+          up[t][r] = rok[r] ? upr[t] : R(0);
+        } else {
+          up[t][r] = pslab[tb + ub + r * rstep + 16 * t] * ((rok[r] && stok[t]) ? R(1) : R(0));
+        }
+      }
+    R wt[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) wt[r] = (R)a.weights[min(nbase + acc_row<R>(g, r), a.N - 1)];
+
+    // forward P_h L_h; the tile of L is kept as the statistic's B operand
+    vec4_t<R> acc[MT];
+#pragma unroll
+    for (int t = 0; t < MT; ++t) acc[t] = vec4_t<R>{R(0), R(0), R(0), R(0)};
+#pragma unroll
+    for (int q = 0; q < KV; ++q) lt[nn * SP + g * KV + q] = arow[q];
+#pragma unroll
+    for (int ks = 0; ks < KV; ++ks)
+#pragma unroll
+      for (int t = 0; t < MT; ++t) acc[t] = mma(arow[ks], Pf[t][ks], acc[t]);
+#pragma unroll
+    for (int t = 0; t < MT; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) mine[acc_row<R>(g, r) * SP + 16 * t + nn] = acc[t][r];
+    lds_barrier();
+    // Ut_h = U_p * (P_other L_other), in the accumulator layout
+    R u[MT][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int m = acc_row<R>(g, r);
+      R d = R(0);
+#pragma unroll
+      for (int t = 0; t < MT; ++t) {
+        u[t][r] = up[t][r] * other[m * SP + 16 * t + nn];
+        d = fma(u[t][r], acc[t][r], d);
+      }
+      if (h == 0) {
+        d = rsum16(d);
+        if (nn == 0) dens[c * kTile + m] = pc * d;
+      }
+    }
+    lds_barrier();
+    // gradient rows: G += (coef Ut)^T L over the tile's 16 patterns
+    R coef[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int m = acc_row<R>(g, r);
+      R den = R(0);
+#pragma unroll
+      for (int cc = 0; cc < 4; ++cc)
+        if (cc < C) den += dens[cc * kTile + m];
+      coef[r] = rok[r] ? div_(wt[r], den) * pc : R(0);
+    }
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      R bv[MT];
+#pragma unroll
+      for (int nt = 0; nt < MT; ++nt) bv[nt] = lt[acc_row<R>(g, ks) * SP + 16 * nt + nn];
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) {
+        const R av = coef[ks] * u[mt][ks];
+#pragma unroll
+        for (int nt = 0; nt < MT; ++nt) G[mt][nt] = mma(av, bv[nt], G[mt][nt]);
+      }
+    }
+    // transpose Ut through this wave's own LDS rows (only this wave reads them)
+#pragma unroll
+    for (int t = 0; t < MT; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) mine[acc_row<R>(g, r) * SP + 16 * t + nn] = u[t][r];
+    vec4_t<R> nu[MT];
+    if (internal) {
+#pragma unroll
+      for (int t = 0; t < MT; ++t) nu[t] = vec4_t<R>{R(0), R(0), R(0), R(0)};
+#pragma unroll
+      for (int ks = 0; ks < KV; ++ks) {
+        const R av = mine[nn * SP + g * KV + ks];  // K index (ks, g) = state g KV + ks
+#pragma unroll
+        for (int t = 0; t < MT; ++t) nu[t] = mma(av, Pt[t][ks], nu[t]);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        R m_ = R(0);
+#pragma unroll
+        for (int t = 0; t < MT; ++t) m_ = fmax_(m_, nu[t][r]);
+        m_ = rmax16(m_);
+        if (nn == 0) mxx[(h * C + c) * kTile + acc_row<R>(g, r)] = m_;
+      }
+    }
+    lds_barrier();
+    if (internal) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = acc_row<R>(g, r);
+        R M = R(0);
+#pragma unroll
+        for (int cc = 0; cc < 4; ++cc)
+          if (cc < C) M = fmax_(M, mxx[(h * C + cc) * kTile + m]);
+        int e = 0;
+        if (M > R(0) && isfinite(M)) frexp_(M, &e);
+#pragma unroll
+        for (int t = 0; t < MT; ++t)
+          if (rok[r] && stok[t]) cslab[tb + ub + r * rstep + 16 * t] = ldexp_(nu[t][r], -e);
+      }
+    }
+    // no barrier: the next tile's first barrier orders this tile's reads of
+    // dens and mxx before their next writes, and each wave rewrites only its
+    // own rows of Yx and Lx before it
+  }
+  if (a.mats_dt != nullptr) {
+    const R* q = a.mats_dt + (((size_t)k * a.B + chh) * C + c) * S * S;
+    double sacc = 0.0;
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < MT; ++nt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = 16 * mt + acc_row<R>(g, r), j = 16 * nt + nn;
+          if (i < S && j < S) sacc = fma((double)G[mt][nt][r], (double)q[i * S + j], sacc);
+        }
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) sacc += __shfl_xor(sacc, o);
+    if (lane == 0) a.gtau[(((size_t)chh * a.K + k) * a.nchunks + chunk) * C + c] = sacc;
+  } else {
+    const size_t css = (size_t)C * S * S;
+    R* o = a.gpart + (((size_t)chh * a.K + k) * a.nchunks + chunk) * css + (size_t)c * S * S;
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < MT; ++nt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = 16 * mt + acc_row<R>(g, r), j = 16 * nt + nn;
+          if (i < S && j < S) o[i * S + j] = G[mt][nt][r];
+        }
+  }
+}
+
 bool use_mfma(int C, int S, int dtype) {
   return (dtype == HX_F32 || dtype == HX_F64) && S != 4 && C >= 1 && C <= 4;
 }
 
 template <typename R>
-static size_t mfma_lds(int C, int S, bool pre) {
+static size_t mfma_lds(int C, int S, bool pre, int tiles_per_chunk) {
   const int MT = S <= 16 ? 1 : (S <= 32 ? 2 : 4);
   const size_t SP = 16 * MT, rb = sizeof(R);
-  if (pre) return (4 * C * kTile * SP + 3 * C * kTile) * rb + 64;
+  // k_pre_mfma_r (MT <= 2) also stages two children's codes for the chunk
+  if (pre)
+    return (4 * C * kTile * SP + 3 * C * kTile) * rb + 64 +
+           (MT <= 2 ? 2 * (size_t)tiles_per_chunk * kTile : 0);
   return (C * kTile * SP + C * kTile) * rb + (size_t)(C * kTile + C * SP) * 8 + 3 * kTile * 4 + 64;
 }
 
@@ -525,8 +824,12 @@ int launch_mfma(bool post, const PassArgs<R>& a, const int32_t* nodes, int width
                 bool root, int S, hipStream_t st) {
   const int MT = S <= 16 ? 1 : (S <= 32 ? 2 : 4);
   const int threads = 2 * a.C * 64;
-  const size_t lds = mfma_lds<R>(a.C, S, !post);
+  const size_t lds = mfma_lds<R>(a.C, S, !post, a.tiles_per_chunk);
   HX_REQUIRE(lds <= 160 * 1024, HX_ERR_UNSUPPORTED, "dense-state kernel LDS exceeds 160 KiB");
+  // k_pre_mfma_r addresses a (node, sample) slab of partials with 32-bit
+  // element offsets, reading up to 16 C S + 64 elements past its end
+  HX_REQUIRE((size_t)(a.N + 16) * a.C * S + 64 < ((size_t)1 << 31), HX_ERR_UNSUPPORTED,
+             "dense-state slab exceeds 31-bit offsets");
   dim3 grid(a.nchunks, a.K, width);
 #define HX_MF(KER, MTV, KVV, TV, RV)                                                         \
   do {                                                                                       \
@@ -554,8 +857,12 @@ int launch_mfma(bool post, const PassArgs<R>& a, const int32_t* nodes, int width
   }
   if (post) {
     HX_MF_KER(k_post_mfma)
+  } else if (MT == 1) {
+    HX_MF_ALL(k_pre_mfma_r, 1, 4)
+  } else if (MT == 2) {
+    if (S <= 20) { HX_MF_ALL(k_pre_mfma_r, 2, 5) } else { HX_MF_ALL(k_pre_mfma_r, 2, 8) }
   } else {
-    HX_MF_KER(k_pre_mfma)
+    HX_MF_ALL(k_pre_mfma, 4, 16)
   }
 #undef HX_MF_KER
 #undef HX_MF_ALL
