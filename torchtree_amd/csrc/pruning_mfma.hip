@@ -80,8 +80,10 @@ __device__ __forceinline__ T row_sum16(T v) {
 // ends with its row's result. Every lane of the wave must be active.
 template <int CTRL>
 __device__ __forceinline__ float dpp_f32(float v) {
-  const int x = __builtin_bit_cast(int, v);
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(x, x, CTRL, 0xF, 0xF, false));
+  // bound_ctrl set: the form the compiler folds into the consuming add / max
+  // (every source lane of these permutes is valid, so it never applies)
+  return __builtin_bit_cast(float,
+                            __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));
 }
 __device__ __forceinline__ float row_max16_dpp(float v) {
   v = fmaxf(v, dpp_f32<0xB1>(v));
@@ -561,9 +563,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int C = a.C;
   R* Yx = reinterpret_cast<R*>(smem_raw);  // [2][C][16][SP]: P L, then Ut transposes
-  R* dens = Yx + 2 * C * kTile * SP;       // [C][16]
-  R* mxx = dens + C * kTile;               // [2][C][16]
-  R* Lx = mxx + 2 * C * kTile;             // [2][C][16][SP]: the child's partial tile
+  R* dens = Yx + 2 * C * kTile * SP;       // [16][4]: per row, one entry per category
+  R* mxx = dens + 4 * kTile;               // [2][16][4]
+  R* Lx = mxx + 8 * kTile;                 // [2][C][16][SP]: the child's partial tile
   uint8_t* ctile = reinterpret_cast<uint8_t*>(Lx + 2 * C * kTile * SP);  // [2][tpc * 16] codes
 
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -617,6 +619,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
     // states 4 KV .. SP - 1 of the tile are never written
     for (int q = lane; q < kTile * SP; q += 64) lt[q] = R(0);
   }
+  // categories C .. 3 of dens and mxx stay zero, so sums and maxima over a
+  // row read all four entries
+  for (int q = threadIdx.x; q < 12 * kTile; q += blockDim.x) dens[q] = R(0);
   bool stok[MT];
 #pragma unroll
   for (int t = 0; t < MT; ++t) stok[t] = 16 * t + nn < S;
@@ -703,7 +708,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
       }
       if (h == 0) {
         d = rsum16(d);
-        if (nn == 0) dens[c * kTile + m] = pc * d;
+        if (nn == 0) dens[m * 4 + c] = pc * d;
       }
     }
     lds_barrier();
@@ -712,10 +717,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int m = acc_row<R>(g, r);
-      R den = R(0);
-#pragma unroll
-      for (int cc = 0; cc < 4; ++cc)
-        if (cc < C) den += dens[cc * kTile + m];
+      const vec4_t<R> dv = *reinterpret_cast<const vec4_t<R>*>(dens + m * 4);
+      const R den = (dv[0] + dv[1]) + (dv[2] + dv[3]);
       coef[r] = rok[r] ? div_(wt[r], den) * pc : R(0);
     }
 #pragma unroll
@@ -751,7 +754,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
         for (int t = 0; t < MT; ++t) m_ = fmax_(m_, nu[t][r]);
         m_ = rmax16(m_);
-        if (nn == 0) mxx[(h * C + c) * kTile + acc_row<R>(g, r)] = m_;
+        if (nn == 0) mxx[(h * kTile + acc_row<R>(g, r)) * 4 + c] = m_;
       }
     }
     lds_barrier();
@@ -759,10 +762,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int m = acc_row<R>(g, r);
-        R M = R(0);
-#pragma unroll
-        for (int cc = 0; cc < 4; ++cc)
-          if (cc < C) M = fmax_(M, mxx[(h * C + cc) * kTile + m]);
+        const vec4_t<R> mv = *reinterpret_cast<const vec4_t<R>*>(mxx + (h * kTile + m) * 4);
+        const R M = fmax_(fmax_(mv[0], mv[1]), fmax_(mv[2], mv[3]));
         int e = 0;
         if (M > R(0) && isfinite(M)) frexp_(M, &e);
 #pragma unroll
@@ -812,10 +813,11 @@ template <typename R>
 static size_t mfma_lds(int C, int S, bool pre, int tiles_per_chunk) {
   const int MT = S <= 16 ? 1 : (S <= 32 ? 2 : 4);
   const size_t SP = 16 * MT, rb = sizeof(R);
-  // k_pre_mfma_r (MT <= 2) also stages two children's codes for the chunk
-  if (pre)
-    return (4 * C * kTile * SP + 3 * C * kTile) * rb + 64 +
-           (MT <= 2 ? 2 * (size_t)tiles_per_chunk * kTile : 0);
+  // k_pre_mfma_r (MT <= 2) pads the per-row category entries to 4 and stages
+  // two children's codes for the chunk
+  if (pre && MT <= 2)
+    return (4 * C * kTile * SP + 12 * kTile) * rb + 64 + 2 * (size_t)tiles_per_chunk * kTile;
+  if (pre) return (4 * C * kTile * SP + 3 * C * kTile) * rb + 64;
   return (C * kTile * SP + C * kTile) * rb + (size_t)(C * kTile + C * SP) * 8 + 3 * kTile * 4 + 64;
 }
 
