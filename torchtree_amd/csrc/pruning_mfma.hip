@@ -319,6 +319,155 @@ __global__ __launch_bounds__(512) void k_post_mfma(PassArgs<R> a,
   }
 }
 
+// Post-order below the root for at most 32 padded states. Every global load
+// of a tile is issued on every path so the compiler's waits are exact: child
+// rows and states are clamped to the child's own last row and state and then
+// zeroed by a multiply (every value read is a written partial), each wave
+// loads the children's scales of the tile's rows (wave 0 stages them in LDS,
+// double-buffered by tile parity so two barriers per tile suffice), and a tip
+// child's codes for the chunk are staged in LDS once. Same products as
+// k_post_mfma.
+template <typename R, int MT, int KV, int TIPS>
+__global__ __launch_bounds__(512) void k_post_mfma_r(PassArgs<R> a,
+                                                     const int32_t* __restrict__ nodes, int S) {
+  static_assert(MT <= 2, "P operands in registers need at most 32 padded states");
+  constexpr int SP = 16 * MT;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int C = a.C;
+  R* Yx = reinterpret_cast<R*>(smem_raw);              // [C][16][SP] child-1 products
+  R* mx = Yx + C * kTile * SP;                         // [16][4] per-row category maxima
+  int* Et = reinterpret_cast<int*>(mx + 4 * kTile);    // [2 parity][2 children][16]
+  uint8_t* ctile = reinterpret_cast<uint8_t*>(Et + 4 * kTile);  // [2][tpc * 16] codes
+
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int h = wave / C, c = wave - h * C;
+  const int nn = lane & 15, g = lane >> 4;
+  const int chunk = blockIdx.x, k = blockIdx.y, v = nodes[blockIdx.z];
+  const int ch0 = a.kids[2 * v], ch1 = a.kids[2 * v + 1];
+  const int chh = h ? ch1 : ch0;
+  const bool internal = chh >= a.T;
+  const int CS = C * S;
+  const size_t slab = (size_t)a.N * CS;
+  const R* cslab = a.partials + ((size_t)(internal ? chh - a.T : 0) * a.K + k) * slab;
+  R* vslab = a.partials + ((size_t)(v - a.T) * a.K + k) * slab;
+  const int t0 = chunk * a.tiles_per_chunk;
+  const int t1 = min(t0 + a.tiles_per_chunk, a.ntiles);
+  const int npat = a.tiles_per_chunk * kTile;
+
+  if constexpr (TIPS == HX_TIPS_CODES) {
+    for (int q = threadIdx.x; q < 2 * npat; q += blockDim.x) {
+      const int hh = q >= npat, m = q - hh * npat, n = t0 * kTile + m;
+      const int chq = hh ? ch1 : ch0;
+      ctile[q] = (chq < a.T && n < a.N) ? a.codes[(size_t)chq * a.N + n] : 0;
+    }
+  }
+  // categories C .. 3 of mx stay zero, so a row's maximum reads all four entries
+  for (int q = threadIdx.x; q < 4 * kTile; q += blockDim.x) mx[q] = R(0);
+
+  R Pf[MT][KV];  // Pf[t][ks] = P[16 t + nn][g KV + ks]
+  {
+    const R* pm = a.mats + (((size_t)k * a.B + chh) * C + c) * S * S;
+#pragma unroll
+    for (int t = 0; t < MT; ++t)
+#pragma unroll
+      for (int ks = 0; ks < KV; ++ks) {
+        const int i = 16 * t + nn, j = g * KV + ks;
+        Pf[t][ks] = (i < S && j < S) ? pm[i * S + j] : R(0);
+      }
+  }
+  bool stok[MT];
+#pragma unroll
+  for (int t = 0; t < MT; ++t) stok[t] = 16 * t + nn < S;
+  const uint32_t rstep = sizeof(R) == 4 ? CS : 4 * CS;
+  const uint32_t ub = (uint32_t)(acc_row<R>(g, 0) * CS + c * S + nn);
+  const uint8_t* mycodes = ctile + h * npat;
+  // children's scales: lane (q = lane & 15, child (lane >> 4) & 1); a tip's reads
+  // land on slab 0 of the scales and are multiplied by 0
+  const int chq = ((lane >> 4) & 1) ? ch1 : ch0;
+  const int32_t* sq = a.scales + ((size_t)(chq >= a.T ? chq - a.T : 0) * a.K + k) * a.N;
+  const float qmask = chq >= a.T ? 1.f : 0.f;
+  lds_barrier();
+
+  for (int tile = t0; tile < t1; ++tile) {
+    const int nbase = tile * kTile;
+    const uint32_t tb = (uint32_t)nbase * (uint32_t)CS;
+    R arow[KV];
+    {
+      const int n = nbase + nn;
+      const int ncl = min(n, a.N - 1);
+      const R rowm = n < a.N ? R(1) : R(0);
+      const R* bp;
+      uint32_t base, last;  // first element and last state of the row read
+      if (internal) {
+        bp = cslab, base = (uint32_t)(ncl * CS + c * S);
+      } else if constexpr (TIPS == HX_TIPS_CODES) {
+        bp = a.table, base = (uint32_t)(mycodes[(tile - t0) * kTile + nn] * S);
+      } else {
+        bp = a.dense + (size_t)chh * a.N * S, base = (uint32_t)(ncl * S);
+      }
+      last = base + S - 1;
+      base += g * KV;
+#pragma unroll
+      for (int q = 0; q < KV; ++q)
+        arow[q] = bp[min(base + q, last)] * ((g * KV + q < S) ? rowm : R(0));
+    }
+    const float sv = (float)sq[min(nbase + (lane & 15), a.N - 1)] * qmask;
+    vec4_t<R> acc[MT];
+#pragma unroll
+    for (int t = 0; t < MT; ++t) acc[t] = vec4_t<R>{R(0), R(0), R(0), R(0)};
+#pragma unroll
+    for (int ks = 0; ks < KV; ++ks)
+#pragma unroll
+      for (int t = 0; t < MT; ++t) acc[t] = mma(arow[ks], Pf[t][ks], acc[t]);
+    // lane (nn, g): acc[t][r] = Y[m = acc_row(g, r)][i = 16 t + nn]
+    if (h == 1) {
+#pragma unroll
+      for (int t = 0; t < MT; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Yx[(c * kTile + acc_row<R>(g, r)) * SP + 16 * t + nn] = acc[t][r];
+    }
+    int* et = Et + (tile & 1) * 2 * kTile;
+    if (wave == 0 && lane < 2 * kTile) et[lane] = (int)sv;
+    lds_barrier();
+    R x[MT][4];
+    if (h == 0) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        R m_ = R(0);
+#pragma unroll
+        for (int t = 0; t < MT; ++t) {
+          x[t][r] = acc[t][r] * Yx[(c * kTile + acc_row<R>(g, r)) * SP + 16 * t + nn];
+          m_ = fmax_(m_, x[t][r]);
+        }
+        m_ = rmax16(m_);
+        if (nn == 0) mx[acc_row<R>(g, r) * 4 + c] = m_;
+      }
+    }
+    lds_barrier();
+    if (h == 0) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = acc_row<R>(g, r), n = nbase + m;
+        const vec4_t<R> mv = *reinterpret_cast<const vec4_t<R>*>(mx + m * 4);
+        const R M = fmax_(fmax_(mv[0], mv[1]), fmax_(mv[2], mv[3]));
+        int e = 0;
+        if (M > R(0) && isfinite(M)) frexp_(M, &e);
+        const int E = et[m] + et[kTile + m] + e;
+        if (n < a.N) {
+#pragma unroll
+          for (int t = 0; t < MT; ++t)
+            if (stok[t]) vslab[tb + ub + r * rstep + 16 * t] = ldexp_(x[t][r], -e);
+          if (c == 0 && nn == 0) a.scales[((size_t)(v - a.T) * a.K + k) * a.N + n] = E;
+        }
+      }
+    }
+    // no third barrier: h = 1 waves rewrite Yx only after the second barrier,
+    // by which the h = 0 waves have read it; mx is rewritten after the next
+    // first barrier; Et alternates between two buffers
+  }
+}
+
 // ----------------------------------------------------------------- pre-order
 template <typename R, int MT, int KV, int TIPS, bool ROOT>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
@@ -857,7 +1006,25 @@ int launch_mfma(bool post, const PassArgs<R>& a, const int32_t* nodes, int width
   } else {                                                                                   \
     HX_MF_ALL(KER, 4, 16)                                                                    \
   }
-  if (post) {
+  if (post && !root && MT <= 2) {
+    // scale tiles, per-row maxima, the chunk's codes
+    const size_t lds_r = (size_t)a.C * kTile * 16 * MT * sizeof(R) + 4 * kTile * sizeof(R) +
+                         4 * kTile * 4 + 2 * (size_t)a.tiles_per_chunk * kTile + 64;
+    HX_REQUIRE(lds_r <= 64 * 1024, HX_ERR_UNSUPPORTED, "dense-state post-order LDS exceeds 64 KiB");
+#define HX_MFR(MTV, KVV)                                                                     \
+  if (tips == HX_TIPS_CODES)                                                                 \
+    k_post_mfma_r<R, MTV, KVV, HX_TIPS_CODES><<<grid, threads, lds_r, st>>>(a, nodes, S);    \
+  else                                                                                       \
+    k_post_mfma_r<R, MTV, KVV, HX_TIPS_DENSE><<<grid, threads, lds_r, st>>>(a, nodes, S);
+    if (MT == 1) {
+      HX_MFR(1, 4)
+    } else if (S <= 20) {
+      HX_MFR(2, 5)
+    } else {
+      HX_MFR(2, 8)
+    }
+#undef HX_MFR
+  } else if (post) {
     HX_MF_KER(k_post_mfma)
   } else if (MT == 1) {
     HX_MF_ALL(k_pre_mfma_r, 1, 4)
