@@ -100,10 +100,30 @@ __global__ void k_reduce_tau(const double* __restrict__ gtau, double* __restrict
   dtau[idx] = t;
 }
 
+// P and P^T of every (sample, branch, category) into zero-padded 64 x 64 tiles:
+// ppad[kbc][0][i][j] = P[i][j], ppad[kbc][1][j][i] = P[i][j]. One block per matrix.
+template <typename R>
+__global__ __launch_bounds__(256) void k_pad_mats(const R* __restrict__ mats, R* __restrict__ ppad,
+                                                  int S) {
+  const size_t m = blockIdx.x;
+  const R* src = mats + m * S * S;
+  R* dst = ppad + m * 2 * 64 * 64;
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int i = e >> 6, j = e & 63;
+    dst[e] = (i < S && j < S) ? src[i * S + j] : R(0);
+    dst[64 * 64 + e] = (i < S && j < S) ? src[j * S + i] : R(0);
+  }
+}
+
 // ------------------------------------------------------------------- host side
 struct WsLayout {
-  size_t partials, tail, scales, rootsums, gpart, total;  // tail: padding after the partials
+  size_t partials, tail, scales, rootsums, gpart, ppad, total;  // tail: padding after the partials
 };
+
+// FP32 gradients at 33..64 states read P and P^T from zero-padded 64 x 64 copies.
+static bool use_ppad(int S, int dtype, int grad_mode) {
+  return grad_mode != 0 && dtype == HX_F32 && S > 32 && S <= 64;
+}
 
 // grad_mode: 0 loglik only, 1 d loglik / d mats, 2 d loglik / d tau.
 static WsLayout ws_layout(const hx_plan_s* plan, int S, int N, int C, int K, int dtype,
@@ -128,6 +148,8 @@ static WsLayout ws_layout(const hx_plan_s* plan, int S, int N, int C, int K, int
   w.gpart = off;
   if (grad_mode == 1) off = align_up(off + B * K * (size_t)t.nchunks * C * S * S * rb, 256);
   if (grad_mode == 2) off = align_up(off + B * K * (size_t)t.nchunks * C * 8, 256);
+  w.ppad = off;
+  if (use_ppad(S, dtype, grad_mode)) off = align_up(off + B * K * (size_t)C * 2 * 64 * 64 * rb, 256);
   w.total = off;
   return w;
 }
@@ -215,6 +237,7 @@ static PassArgs<R> make_args(const CallArgs& c) {
   a.gpart = reinterpret_cast<R*>(base + w.gpart);
   a.mats_dt = c.grad_mode == 2 ? static_cast<const R*>(c.mats_dt) : nullptr;
   a.gtau = reinterpret_cast<double*>(base + w.gpart);
+  a.ppad = use_ppad(c.S, c.dtype, c.grad_mode) ? reinterpret_cast<R*>(base + w.ppad) : nullptr;
   return a;
 }
 
@@ -256,6 +279,10 @@ static int run_pre(const CallArgs& c, double* out, hipStream_t st) {
   const PassArgs<R> a = make_args<R>(c);
   const hx_plan_s* plan = c.plan;
   const int D = (int)plan->pre_off.size() - 1;
+  if (a.ppad != nullptr) {
+    k_pad_mats<R><<<(unsigned)((size_t)c.K * a.B * c.C), 256, 0, st>>>(a.mats, a.ppad, c.S);
+    HX_HIP_TRY(hipGetLastError());
+  }
   if constexpr (std::is_same<R, float>::value) {
     if (use_pre_walk(c.S, c.C, 4, c.N, c.K)) {
       const int rc = launch_pre_walk(a, plan->d_prew, plan->I, c.tip_kind, st);

@@ -28,6 +28,7 @@ struct PassArgs {
   R* gpart;               // [B][K][nchunks][C][S][S]     (d loglik / d mats mode)
   const R* mats_dt;       // [K][B][C][S][S] dP/dtau; non-null selects d loglik / d tau mode
   double* gtau;           // [B][K][nchunks][C]            (d loglik / d tau mode)
+  R* ppad;                // [K][B][C][2][64][64]: P and P^T zero-padded (FP32, 33..64 states)
 };
 
 constexpr double kLn2 = 0.69314718055994530942;

@@ -695,7 +695,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
 }
 
 // Pre-order for at most 32 padded states with both P operands in registers
-// (the forward's P^T and the transpose's P) for the whole chunk. Each tile
+// (the forward's P^T and the transpose's P) for the whole chunk; at 64 padded
+// states (FP32 only) the 64 gradient accumulators leave no room for them, and
+// each tile reads them as aligned 16-byte loads from zero-padded 64 x 64 copies
+// of P and P^T (ppad). Each tile
 // issues the same global loads on every path, so the compiler's waits are
 // exact: rows past N and states past S of the workspace partials read the
 // padding that follows them, reads of caller buffers (codes, tables, dense
@@ -705,9 +708,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
 // Same products and barriers as k_pre_mfma.
 template <typename R, int MT, int KV, int TIPS, bool ROOT>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
-    sizeof(R) == 4 ? 4 : 1))) void k_pre_mfma_r(PassArgs<R> a, const int32_t* __restrict__ nodes,
-                                                int S) {
-  static_assert(MT <= 2, "P operands in registers need at most 32 padded states");
+    sizeof(R) == 4 && MT <= 2 ? 4 : 1))) void k_pre_mfma_r(PassArgs<R> a,
+                                                           const int32_t* __restrict__ nodes, int S) {
+  static_assert(MT <= 2 || (sizeof(R) == 4 && KV == 16), "64 padded states: FP32, 16 steps");
+  constexpr bool kPReg = MT <= 2;
   constexpr int SP = 16 * MT;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int C = a.C;
@@ -745,16 +749,23 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
 
   // Pf[t][ks] = P[16 t + nn][g KV + ks] (forward), Pt[t][ks] = P[g KV + ks][16 t + nn]
   // (transpose)
-  R Pf[MT][KV], Pt[MT][KV];
+  R Pf[kPReg ? MT : 1][kPReg ? KV : 1], Pt[kPReg ? MT : 1][kPReg ? KV : 1];
+  if constexpr (kPReg) {
 #pragma unroll
-  for (int t = 0; t < MT; ++t)
+    for (int t = 0; t < MT; ++t)
 #pragma unroll
-    for (int ks = 0; ks < KV; ++ks) {
-      const int i = 16 * t + nn, j = g * KV + ks;
-      const bool ok = i < S && j < S;
-      Pf[t][ks] = ok ? pm[i * S + j] : R(0);
-      Pt[t][ks] = ok ? pm[j * S + i] : R(0);
-    }
+      for (int ks = 0; ks < KV; ++ks) {
+        const int i = 16 * t + nn, j = g * KV + ks;
+        const bool ok = i < S && j < S;
+        Pf[t][ks] = ok ? pm[i * S + j] : R(0);
+        Pt[t][ks] = ok ? pm[j * S + i] : R(0);
+      }
+  }
+  // padded copies: [0][i][j] = P[i][j], [1][j][i] = P[i][j]; lane (nn, g) reads row
+  // 16 t + nn, states g KV .. g KV + 15
+  const R* ppf = kPReg ? nullptr
+                       : a.ppad + (((size_t)k * a.B + chh) * C + c) * 2 * 64 * 64 + nn * 64 + g * KV;
+  const R* ppt = ppf + 64 * 64;
   const R pc = a.props[(size_t)k * a.props_ks + c];
   vec4_t<R> G[MT][MT];  // G[Mt][Nt] lane (nn, g) reg r = dlogL/dP[16 Mt + acc_row(g, r)][16 Nt + nn]
 #pragma unroll
@@ -835,10 +846,21 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
     for (int t = 0; t < MT; ++t) acc[t] = vec4_t<R>{R(0), R(0), R(0), R(0)};
 #pragma unroll
     for (int q = 0; q < KV; ++q) lt[nn * SP + g * KV + q] = arow[q];
+    if constexpr (kPReg) {
 #pragma unroll
-    for (int ks = 0; ks < KV; ++ks)
+      for (int ks = 0; ks < KV; ++ks)
 #pragma unroll
-      for (int t = 0; t < MT; ++t) acc[t] = mma(arow[ks], Pf[t][ks], acc[t]);
+        for (int t = 0; t < MT; ++t) acc[t] = mma(arow[ks], Pf[t][ks], acc[t]);
+    } else {
+#pragma unroll
+      for (int t = 0; t < MT; ++t)
+#pragma unroll
+        for (int q = 0; q < KV; q += 4) {
+          const vec4_t<R> pv = *reinterpret_cast<const vec4_t<R>*>(ppf + 16 * t * 64 + q);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) acc[t] = mma(arow[q + e], pv[e], acc[t]);
+        }
+    }
 #pragma unroll
     for (int t = 0; t < MT; ++t)
 #pragma unroll
@@ -891,11 +913,22 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
     if (internal) {
 #pragma unroll
       for (int t = 0; t < MT; ++t) nu[t] = vec4_t<R>{R(0), R(0), R(0), R(0)};
+      if constexpr (kPReg) {
 #pragma unroll
-      for (int ks = 0; ks < KV; ++ks) {
-        const R av = mine[nn * SP + g * KV + ks];  // K index (ks, g) = state g KV + ks
+        for (int ks = 0; ks < KV; ++ks) {
+          const R av = mine[nn * SP + g * KV + ks];  // K index (ks, g) = state g KV + ks
 #pragma unroll
-        for (int t = 0; t < MT; ++t) nu[t] = mma(av, Pt[t][ks], nu[t]);
+          for (int t = 0; t < MT; ++t) nu[t] = mma(av, Pt[t][ks], nu[t]);
+        }
+      } else {
+#pragma unroll
+        for (int t = 0; t < MT; ++t)
+#pragma unroll
+          for (int q = 0; q < KV; q += 4) {
+            const vec4_t<R> pv = *reinterpret_cast<const vec4_t<R>*>(ppt + 16 * t * 64 + q);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) nu[t] = mma(mine[nn * SP + g * KV + q + e], pv[e], nu[t]);
+          }
       }
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -964,7 +997,7 @@ static size_t mfma_lds(int C, int S, bool pre, int tiles_per_chunk) {
   const size_t SP = 16 * MT, rb = sizeof(R);
   // k_pre_mfma_r (MT <= 2) pads the per-row category entries to 4 and stages
   // two children's codes for the chunk
-  if (pre && MT <= 2)
+  if (pre && (MT <= 2 || rb == 4))
     return (4 * C * kTile * SP + 12 * kTile) * rb + 64 + 2 * (size_t)tiles_per_chunk * kTile;
   if (pre) return (4 * C * kTile * SP + 3 * C * kTile) * rb + 64;
   return (C * kTile * SP + C * kTile) * rb + (size_t)(C * kTile + C * SP) * 8 + 3 * kTile * 4 + 64;
@@ -1030,6 +1063,9 @@ int launch_mfma(bool post, const PassArgs<R>& a, const int32_t* nodes, int width
     HX_MF_ALL(k_pre_mfma_r, 1, 4)
   } else if (MT == 2) {
     if (S <= 20) { HX_MF_ALL(k_pre_mfma_r, 2, 5) } else { HX_MF_ALL(k_pre_mfma_r, 2, 8) }
+  } else if constexpr (sizeof(R) == 4) {
+    HX_REQUIRE(a.ppad != nullptr, HX_ERR_INVALID_ARGUMENT, "padded P copies missing");
+    HX_MF_ALL(k_pre_mfma_r, 4, 16)
   } else {
     HX_MF_ALL(k_pre_mfma, 4, 16)
   }
