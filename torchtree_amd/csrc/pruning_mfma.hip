@@ -319,7 +319,7 @@ __global__ __launch_bounds__(512) void k_post_mfma(PassArgs<R> a,
   }
 }
 
-// Post-order below the root for at most 32 padded states. Every global load
+// Post-order below the root (P in registers at up to 64 padded states). Every global load
 // of a tile is issued on every path so the compiler's waits are exact: child
 // rows and states are clamped to the child's own last row and state and then
 // zeroed by a multiply (every value read is a written partial), each wave
@@ -330,7 +330,6 @@ __global__ __launch_bounds__(512) void k_post_mfma(PassArgs<R> a,
 template <typename R, int MT, int KV, int TIPS>
 __global__ __launch_bounds__(512) void k_post_mfma_r(PassArgs<R> a,
                                                      const int32_t* __restrict__ nodes, int S) {
-  static_assert(MT <= 2, "P operands in registers need at most 32 padded states");
   constexpr int SP = 16 * MT;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int C = a.C;
@@ -1039,11 +1038,12 @@ int launch_mfma(bool post, const PassArgs<R>& a, const int32_t* nodes, int width
   } else {                                                                                   \
     HX_MF_ALL(KER, 4, 16)                                                                    \
   }
-  if (post && !root && MT <= 2) {
+  if (post && !root) {
     // scale tiles, per-row maxima, the chunk's codes
     const size_t lds_r = (size_t)a.C * kTile * 16 * MT * sizeof(R) + 4 * kTile * sizeof(R) +
                          4 * kTile * 4 + 2 * (size_t)a.tiles_per_chunk * kTile + 64;
     HX_REQUIRE(lds_r <= 64 * 1024, HX_ERR_UNSUPPORTED, "dense-state post-order LDS exceeds 64 KiB");
+    // MT 4: C <= 4, so C x 16 x 64 products take at most 16 KiB (32 KiB in FP64)
 #define HX_MFR(MTV, KVV)                                                                     \
   if (tips == HX_TIPS_CODES)                                                                 \
     k_post_mfma_r<R, MTV, KVV, HX_TIPS_CODES><<<grid, threads, lds_r, st>>>(a, nodes, S);    \
@@ -1051,6 +1051,8 @@ int launch_mfma(bool post, const PassArgs<R>& a, const int32_t* nodes, int width
     k_post_mfma_r<R, MTV, KVV, HX_TIPS_DENSE><<<grid, threads, lds_r, st>>>(a, nodes, S);
     if (MT == 1) {
       HX_MFR(1, 4)
+    } else if (MT == 4) {
+      HX_MFR(4, 16)
     } else if (S <= 20) {
       HX_MFR(2, 5)
     } else {
