@@ -61,7 +61,9 @@ def bytes_per_pattern(T, C, S, K, dtype):
 def fixed_bytes(T, C, S, K, dtype):
     rb = 8 if dtype == F64 else 4
     B = 2 * T - 2
-    return 2 * K * B * C * S * S * rb + 3 * K * B * C * 8 + (1 << 20)
+    # FP32 gradients at 33..64 states also keep zero-padded 64 x 64 copies of P and P^T
+    pad = K * B * C * 2 * 64 * 64 * 4 if (rb == 4 and 32 < S <= 64) else 0
+    return 2 * K * B * C * S * S * rb + 3 * K * B * C * 8 + pad + (1 << 20)
 
 
 class ResidentJob:
