@@ -63,6 +63,46 @@ def test_dropin_wide(gpu, S, C, dtype, rl, rg):
         assert err < rg, (name, err)
 
 
+@pytest.mark.parametrize("S,C,T", [(20, 4, 10), (61, 4, 10), (20, 3, 25), (61, 2, 25)])
+@pytest.mark.parametrize("dtype,rl,rg", [(F64, 1e-12, 1e-10), (torch.float32, 1e-6, 1e-5)])
+def test_dropin_wide_stale_workspace(gpu, S, C, T, dtype, rl, rg, monkeypatch):
+    """Every workspace byte starts as 0xFF (NaN in both precisions), so memory the kernels did
+    not write cannot reach a result unnoticed. The dense pre-order reads past the rows of the
+    last 16-pattern tile (N = 37); those reads land on other partials, on the padding after
+    them or on the head of the root's first slab, which the post-order never stores and
+    zeroes at its end. T = 10 and 25 put the root at different slots."""
+    print(f"case: wide drop-in, NaN-filled workspace S={S} C={C} T={T} {dtype}", flush=True)
+    from torchtree_amd import likelihood as hl
+
+    real_empty = torch.empty
+
+    def empty_nan(*args, **kwargs):
+        t = real_empty(*args, **kwargs)
+        if t.dtype == torch.uint8 and t.is_cuda:
+            t.fill_(0xFF)
+        return t
+
+    N, K = 37, 2
+    post, tips, codes, w, pi, ex, mats, rates, props, bl = _random_case(S, T, N, K, C, S + 3 * T + C)
+    m = mats.detach().clone().requires_grad_(True)
+    f = pi.reshape(1, -1).clone().requires_grad_(True)
+    p = props.reshape(-1, 1, 1).clone().requires_grad_(True)
+    ref = opruning.treelik_discrete_rescaled(tips + [None] * (T - 1), w, post, m, f, p)
+    ref.sum().backward()
+    m2 = mats.detach().to(gpu, dtype).requires_grad_(True)
+    f2 = pi.reshape(1, -1).to(gpu, dtype).requires_grad_(True)
+    p2 = props.reshape(-1, 1, 1).to(gpu, dtype).requires_grad_(True)
+    monkeypatch.setattr(torch, "empty", empty_nan)
+    got = hl.calculate_treelikelihood_discrete(tips + [None] * (T - 1), w, post, m2, f2, p2)
+    got.sum().backward()
+    monkeypatch.setattr(torch, "empty", real_empty)
+    rel = float(((got.detach().double().cpu() - ref.detach()).abs() / ref.detach().abs()).max())
+    assert rel < rl, rel
+    for a, b, name in ((m2.grad, m.grad, "mats"), (f2.grad, f.grad, "freqs"), (p2.grad, p.grad, "props")):
+        err = float((a.double().cpu() - b).norm() / b.norm())
+        assert err < rg, (name, err)
+
+
 @pytest.mark.parametrize("dtype,rl,rg", [(F64, 1e-12, 1e-9), (torch.float32, 1e-6, 1e-5)])
 def test_engine_aa(gpu, dtype, rl, rg):
     print(f"case: engine AA S=20 gamma4 {dtype}", flush=True)
