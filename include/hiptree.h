@@ -178,6 +178,13 @@ int hx_job_destroy(hx_job_t job);
 size_t hx_job_resident_bytes(hx_job_t job);
 /* Workspace of one hx_job_eval with K samples (with or without the gradient). */
 size_t hx_job_workspace_bytes(hx_job_t job, int K, int want_grad);
+/* What hx_job_workspace_bytes reports for a job of this shape (taxa, states,
+ * pattern slice, categories, dtype), computed on the host without creating
+ * the job: for choosing a slice before registration. No counterpart in the
+ * reference, which allocates per call through torch. Returns 0 for invalid
+ * arguments. */
+size_t hx_job_workspace_bytes_for(int taxa, int S, int slice_patterns, int C, int K, int dtype,
+                                  int want_grad);
 /* One request: bls [K][B] (device, double; unrooted trees carry 0 in slot
  * 2T-3) -> out_loglik [K] and, if out_dbls is not NULL, out_dbls [K][B]
  * (device, double). Pattern slices run in order on `stream`; results do not

@@ -119,3 +119,20 @@ def test_served_walk_matches_oracle(gpu):
     g = b.grad[:, :-1]  # slot 2T-3 is a padded (fixed) length
     err = float((req.dbls.cpu()[:, :-1] - g).norm() / g.norm())
     assert err < 1e-5, err
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, F64])
+def test_workspace_size_without_a_job(gpu, dtype):
+    """hx_job_workspace_bytes_for (host only, used to choose pattern slices before a job
+    exists) gives exactly what the registered job asks for, at every state count and K."""
+    print(f"case: job workspace from its shape {dtype}", flush=True)
+    from torchtree_amd import _lib
+
+    lib = _lib.load()
+    code = _lib.HX_F64 if dtype == F64 else _lib.HX_F32
+    specs, jobs = _jobs(gpu, dtype, slice_for=lambda sp: 100)
+    for sp, job in zip(specs, jobs):
+        for K in (1, 8, 32):
+            for grad in (0, 1):
+                got = int(lib.hx_job_workspace_bytes_for(job.T, job.S, job.slice, job.C, K, code, grad))
+                assert got == job.workspace_bytes(K, bool(grad)), (sp["name"], K, grad)

@@ -185,3 +185,35 @@ def test_integration_install_rejects_other_modules():
     with pytest.raises(AttributeError):
         integration.install(mod)
     assert mod.calculate_treelikelihood == 1
+
+
+def test_slice_for_fits_the_stream_workspace():
+    """Server.slice_for picks a pattern slice whose job workspace, as the library computes it on
+    the host (hx_job_workspace_bytes_for), fits one stream: four-state, amino-acid and codon
+    jobs, both precisions (the FP32 codon gradient adds padded P copies that do not shrink
+    with the slice)."""
+    from types import SimpleNamespace
+
+    from torchtree_amd import _lib, serving
+
+    lib = _lib.load()
+    assert lib.hx_job_workspace_bytes_for(1, 4, 10, 4, 1, _lib.HX_F32, 1) == 0
+    assert lib.hx_job_workspace_bytes_for(16, 4, 10, 4, 1, 7, 1) == 0
+    for T, N, C, S, dt in ((512, 100000, 4, 4, torch.float32), (17, 150, 4, 4, torch.float32),
+                           (256, 20000, 4, 20, torch.float32), (1024, 100000, 4, 61, torch.float32),
+                           (1024, 100000, 4, 61, torch.float64)):
+        code = _lib.HX_F64 if dt == torch.float64 else _lib.HX_F32
+        full = lib.hx_job_workspace_bytes_for(T, S, N, C, 32, code, 1)
+        assert full > lib.hx_job_workspace_bytes_for(T, S, N, C, 32, code, 0) > 0
+        one = lib.hx_job_workspace_bytes_for(T, S, 1, C, 32, code, 1)
+        for frac in (0.3, 0.7, 1.0):
+            srv = SimpleNamespace(ws_bytes=int(full * frac))
+            if srv.ws_bytes < one:
+                with pytest.raises(serving.BudgetError):
+                    serving.Server.slice_for(srv, T, N, C, S, dt)
+                continue
+            sl = serving.Server.slice_for(srv, T, N, C, S, dt)
+            assert 1 <= sl <= N
+            assert lib.hx_job_workspace_bytes_for(T, S, sl, C, 32, code, 1) <= srv.ws_bytes
+            if frac == 1.0:
+                assert sl == N

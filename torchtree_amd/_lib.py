@@ -61,6 +61,7 @@ _SIGNATURES = {
     "hx_job_destroy": (_c_int, [_vp]),
     "hx_job_resident_bytes": (_c_size, [_vp]),
     "hx_job_workspace_bytes": (_c_size, [_vp, _c_int, _c_int]),
+    "hx_job_workspace_bytes_for": (_c_size, [_c_int] * 7),
     "hx_job_eval": (_c_int, [_vp, _c_int, _dp, _dp, _dp, _vp, _c_size, _vp]),
     "hx_selftest_mfma_layout": (_c_int, [ctypes.POINTER(_c_int), _vp]),
 }

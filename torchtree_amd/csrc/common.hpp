@@ -82,6 +82,10 @@ inline bool use_pre_walk(int S, int C, int rb, int N, int K) {
   return S == 4 && rb == 4 && C <= kPreWalkMaxC && groups >= kPreWalkMinGroups;
 }
 
+// hx_workspace_bytes for a tree of `taxa` taxa (the layout needs no plan);
+// defined in pruning.hip.
+size_t workspace_bytes_for(int taxa, int S, int N, int C, int K, int dtype, int grad_mode);
+
 inline Tiling make_tiling(int N, int C, int K, int S, int rb) {
   Tiling t;
   t.cp = next_pow2(C < 1 ? 1 : C);

@@ -68,10 +68,13 @@ struct JobWs {
   size_t mats, mats_dt, prune, prune_bytes, loglik_tmp, dtau_tmp, dtau_acc, total;
 };
 
-static JobWs job_ws(const hx_job_s* j, int K, int grad) {
-  const size_t rb = j->dtype == HX_F64 ? 8 : 4;
-  const size_t nm = (size_t)K * j->B * j->C * j->S * j->S;
-  const size_t nt = (size_t)K * j->B * j->C;
+// Workspace of one evaluation: P (and dP/dtau), one pattern slice's pruning
+// workspace, and the per-request sums.
+static JobWs job_ws_shape(int taxa, int S, int slice, int C, int K, int dtype, int grad) {
+  const size_t rb = dtype == HX_F64 ? 8 : 4;
+  const size_t B = 2 * (size_t)taxa - 2;
+  const size_t nm = (size_t)K * B * C * S * S;
+  const size_t nt = (size_t)K * B * C;
   JobWs w{};
   size_t off = 0;
   w.mats = off;
@@ -81,8 +84,7 @@ static JobWs job_ws(const hx_job_s* j, int K, int grad) {
     off = align_up(off + nm * rb, 256);
   }
   w.prune = off;
-  w.prune_bytes = hx_workspace_bytes(j->plan, j->S, j->slice, j->C, K, j->dtype,
-                                     grad ? HX_GRAD_TAU : HX_GRAD_NONE);
+  w.prune_bytes = workspace_bytes_for(taxa, S, slice, C, K, dtype, grad ? HX_GRAD_TAU : HX_GRAD_NONE);
   off = align_up(off + w.prune_bytes, 256);
   w.loglik_tmp = off;
   off = align_up(off + (size_t)K * 8, 256);
@@ -94,6 +96,10 @@ static JobWs job_ws(const hx_job_s* j, int K, int grad) {
   }
   w.total = off;
   return w;
+}
+
+static JobWs job_ws(const hx_job_s* j, int K, int grad) {
+  return job_ws_shape(j->plan->T, j->S, j->slice, j->C, K, j->dtype, grad);
 }
 
 }  // namespace hx
@@ -216,6 +222,13 @@ size_t hx_job_resident_bytes(hx_job_t job) { return job ? job->resident : 0; }
 size_t hx_job_workspace_bytes(hx_job_t job, int K, int want_grad) {
   if (!job || K < 1) return 0;
   return hx::job_ws(job, K, want_grad).total;
+}
+
+size_t hx_job_workspace_bytes_for(int taxa, int S, int slice_patterns, int C, int K, int dtype,
+                                  int want_grad) {
+  if (taxa < 2 || S < 2 || slice_patterns < 1 || C < 1 || K < 1) return 0;
+  if (dtype != HX_F32 && dtype != HX_F64) return 0;
+  return hx::job_ws_shape(taxa, S, slice_patterns, C, K, dtype, want_grad).total;
 }
 
 int hx_job_eval(hx_job_t job, int K, const double* bls, double* out_loglik, double* out_dbls,

@@ -126,11 +126,11 @@ static bool use_ppad(int S, int dtype, int grad_mode) {
 }
 
 // grad_mode: 0 loglik only, 1 d loglik / d mats, 2 d loglik / d tau.
-static WsLayout ws_layout(const hx_plan_s* plan, int S, int N, int C, int K, int dtype,
-                          int grad_mode) {
+// The layout depends on the tree only through its taxon count.
+static WsLayout ws_layout_t(int taxa, int S, int N, int C, int K, int dtype, int grad_mode) {
   const size_t rb = dtype == HX_F64 ? 8 : 4;
   const Tiling t = make_tiling(N, C, K, S, (int)rb);
-  const size_t I = plan->I, B = 2 * (size_t)plan->T - 2;
+  const size_t I = (size_t)taxa - 1, B = 2 * (size_t)taxa - 2;
   WsLayout w;
   size_t off = 0;
   w.partials = off;
@@ -141,7 +141,7 @@ static WsLayout ws_layout(const hx_plan_s* plan, int S, int N, int C, int K, int
   const size_t tail = S == 4 ? 0 : (16 * (size_t)C * S + 64) * rb;
   off = align_up(w.tail + tail, 256);
   w.scales = off;  // S = 4: 64-byte-aligned copy of the tip codes [T][ntiles * 64]
-  if (S == 4) off = align_up(off + (size_t)plan->T * t.ntiles * 64, 256);
+  if (S == 4) off = align_up(off + (size_t)taxa * t.ntiles * 64, 256);
   else off = align_up(off + I * K * (size_t)N * 4, 256);
   w.rootsums = off;
   off = align_up(off + (size_t)K * t.ntiles * (1 + S + C) * 8, 256);
@@ -152,6 +152,15 @@ static WsLayout ws_layout(const hx_plan_s* plan, int S, int N, int C, int K, int
   if (use_ppad(S, dtype, grad_mode)) off = align_up(off + B * K * (size_t)C * 2 * 64 * 64 * rb, 256);
   w.total = off;
   return w;
+}
+
+static WsLayout ws_layout(const hx_plan_s* plan, int S, int N, int C, int K, int dtype,
+                          int grad_mode) {
+  return ws_layout_t(plan->T, S, N, C, K, dtype, grad_mode);
+}
+
+size_t workspace_bytes_for(int taxa, int S, int N, int C, int K, int dtype, int grad_mode) {
+  return ws_layout_t(taxa, S, N, C, K, dtype, grad_mode).total;
 }
 
 template <typename R>

@@ -53,19 +53,6 @@ def _host_eigen(q_norm: torch.Tensor, freqs: torch.Tensor):
             np.ascontiguousarray(right.numpy()))
 
 
-def bytes_per_pattern(T, C, S, K, dtype):
-    rb = 8 if dtype == F64 else 4
-    return (T - 1) * K * (C * S * rb + 4)
-
-
-def fixed_bytes(T, C, S, K, dtype):
-    rb = 8 if dtype == F64 else 4
-    B = 2 * T - 2
-    # FP32 gradients at 33..64 states also keep zero-padded 64 x 64 copies of P and P^T
-    pad = K * B * C * 2 * 64 * 64 * 4 if (rb == 4 and 32 < S <= 64) else 0
-    return 2 * K * B * C * S * S * rb + 3 * K * B * C * 8 + pad + (1 << 20)
-
-
 class ResidentJob:
     """One alignment resident on the device (``hx_job_create``)."""
 
@@ -103,7 +90,7 @@ class ResidentJob:
             torch.cuda.set_device(prev)
         _lib.check(rc, "hx_job_create")
         self.handle = h
-        self.slice = slice_patterns if slice_patterns else self.N
+        self.slice = min(slice_patterns, self.N) if slice_patterns > 0 else self.N  # as hx_job_create
         self.resident_bytes = int(lib.hx_job_resident_bytes(h))
 
     def workspace_bytes(self, K, grad=True):
@@ -243,12 +230,30 @@ class Server:
         self.rejected = 0
 
     def slice_for(self, T, N, C, S, dtype, max_k=32):
-        """Largest pattern slice whose max-K workspace fits one stream."""
-        room = self.ws_bytes - fixed_bytes(T, C, S, max_k, dtype)
-        per = bytes_per_pattern(T, C, S, max_k, dtype)
-        if room <= per:
+        """Largest pattern slice whose max-K gradient workspace fits one stream.
+
+        Sizes come from the library (hx_job_workspace_bytes_for), so they match what a
+        registered job will ask for. The size is not monotone in the slice everywhere (the
+        four-state gradient switches kernels with the workgroup count), so this bisects
+        keeping a slice that fits."""
+        lib = _lib.load()
+        code = _lib.HX_F64 if dtype == F64 else _lib.HX_F32
+
+        def need(n):
+            return int(lib.hx_job_workspace_bytes_for(T, S, n, C, max_k, code, 1))
+
+        if need(1) > self.ws_bytes:
             raise BudgetError("one pattern of this job does not fit a stream workspace")
-        return int(min(N, max(1, 0.9 * room // per)))
+        if need(N) <= self.ws_bytes:
+            return N
+        lo, hi = 1, N
+        while hi - lo > 1:
+            mid = (lo + hi) // 2
+            if need(mid) <= self.ws_bytes:
+                lo = mid
+            else:
+                hi = mid
+        return lo
 
     def register(self, name, job):
         if self.resident + job.resident_bytes > self.resident_cap:
