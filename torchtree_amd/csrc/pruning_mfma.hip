@@ -85,11 +85,21 @@ __device__ __forceinline__ float dpp_f32(float v) {
   return __builtin_bit_cast(float,
                             __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));
 }
+// The maximum is taken on the bit patterns as signed integers, which order
+// non-negative floats as floats do and put every negative value and -0 below
+// +0; the callers' inputs are maxima started at +0, so the result is the
+// float maximum (and the integer max folds the permute into one instruction).
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int x) {
+  return __builtin_amdgcn_mov_dpp(x, CTRL, 0xF, 0xF, true);
+}
 __device__ __forceinline__ float row_max16_dpp(float v) {
-  v = fmaxf(v, dpp_f32<0xB1>(v));
-  v = fmaxf(v, dpp_f32<0x4E>(v));
-  v = fmaxf(v, dpp_f32<0x141>(v));
-  return fmaxf(v, dpp_f32<0x140>(v));
+  int x = __builtin_bit_cast(int, v);
+  x = max(x, dpp_i32<0xB1>(x));
+  x = max(x, dpp_i32<0x4E>(x));
+  x = max(x, dpp_i32<0x141>(x));
+  x = max(x, dpp_i32<0x140>(x));
+  return __builtin_bit_cast(float, x);
 }
 __device__ __forceinline__ float row_sum16_dpp(float v) {
   v += dpp_f32<0xB1>(v);
