@@ -703,6 +703,32 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
   }
 }
 
+// Child rows of one tile for k_pre_mfma_r: states g KV .. g KV + KV - 1 of
+// pattern tile * 16 + nn, every load issued (clamped caller buffers, padded
+// workspace) and out-of-range values zeroed by a multiply.
+template <typename R, int KV, int TIPS>
+__device__ __forceinline__ void pre_rows(const PassArgs<R>& a, bool internal, const R* cslab,
+                                         int chh, const uint8_t* mycodes, int tile, int t0,
+                                         int nn, int g, int S, int CS, uint32_t cb,
+                                         R (&arow)[KV]) {
+  const int nbase = tile * kTile;
+  const bool nok = nbase + nn < a.N;
+  const R* bp;
+  uint32_t base, lim;
+  if (internal) {
+    bp = cslab, base = (uint32_t)nbase * (uint32_t)CS + cb, lim = 0xffffffffu;
+  } else if constexpr (TIPS == HX_TIPS_CODES) {
+    const int code = mycodes[(tile - t0) * kTile + nn];
+    bp = a.table, base = (uint32_t)(code * S + g * KV), lim = (uint32_t)(a.ncodes * S - 1);
+  } else {
+    bp = a.dense + (size_t)chh * a.N * S;
+    base = (uint32_t)(min(nbase + nn, a.N - 1) * S + g * KV), lim = (uint32_t)(a.N * S - 1);
+  }
+#pragma unroll
+  for (int q = 0; q < KV; ++q)
+    arow[q] = bp[min(base + q, lim)] * ((nok && g * KV + q < S) ? R(1) : R(0));
+}
+
 // Pre-order for at most 32 padded states with both P operands in registers
 // (the forward's P^T and the transpose's P) for the whole chunk; at 64 padded
 // states (FP32 only) the 64 gradient accumulators leave no room for them, and
@@ -728,7 +754,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
   R* dens = Yx + 2 * C * kTile * SP;       // [16][4]: per row, one entry per category
   R* mxx = dens + 4 * kTile;               // [2][16][4]
   R* Lx = mxx + 8 * kTile;                 // [2][C][16][SP]: the child's partial tile
-  uint8_t* ctile = reinterpret_cast<uint8_t*>(Lx + 2 * C * kTile * SP);  // [2][tpc * 16] codes
+  constexpr int kPtS = SP + 1;             // row stride of the P tiles (no bank conflicts)
+  R* PtL = Lx + 2 * C * kTile * SP;        // [2][C][4 KV][SP + 1]: the transpose's P (MT <= 2)
+  uint8_t* ctile = reinterpret_cast<uint8_t*>(PtL + (MT <= 2 ? 2 * C * 4 * KV * kPtS : 0));
+  // [2][tpc * 16] codes
 
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -758,7 +787,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
 
   // Pf[t][ks] = P[16 t + nn][g KV + ks] (forward), Pt[t][ks] = P[g KV + ks][16 t + nn]
   // (transpose)
-  R Pf[kPReg ? MT : 1][kPReg ? KV : 1], Pt[kPReg ? MT : 1][kPReg ? KV : 1];
+  // the transpose's operand P[g KV + ks][16 t + nn] lives in this wave's LDS tile
+  R Pf[kPReg ? MT : 1][kPReg ? KV : 1];
+  R* ptw = PtL + (size_t)(h * C + c) * 4 * KV * kPtS;
   if constexpr (kPReg) {
 #pragma unroll
     for (int t = 0; t < MT; ++t)
@@ -767,7 +798,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
         const int i = 16 * t + nn, j = g * KV + ks;
         const bool ok = i < S && j < S;
         Pf[t][ks] = ok ? pm[i * S + j] : R(0);
-        Pt[t][ks] = ok ? pm[j * S + i] : R(0);
+        ptw[j * kPtS + i] = ok ? pm[j * S + i] : R(0);
       }
   }
   // padded copies: [0][i][j] = P[i][j], [1][j][i] = P[i][j]; lane (nn, g) reads row
@@ -808,29 +839,16 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
   const uint32_t cb = (uint32_t)(nn * CS + c * S + g * KV);
   const uint8_t* mycodes = ctile + h * npat;
   lds_barrier();
+  // the first tile's child rows; each tile then issues the next tile's rows
+  // after its forward product, so their latency overlaps the rest of the tile
+  R arow[KV];
+  pre_rows<R, KV, TIPS>(a, internal, cslab, chh, mycodes, t0, t0, nn, g, S, CS, cb, arow);
 
   for (int tile = t0; tile < t1; ++tile) {
     const int nbase = tile * kTile;
     const uint32_t tb = (uint32_t)nbase * (uint32_t)CS;
-    // child rows first (the forward waits for them alone), then the parent's
-    // adjoint, then the weights
-    R arow[KV];
-    {
-      const bool nok = nbase + nn < a.N;
-      const R* bp;
-      uint32_t base, lim;
-      if (internal) {
-        bp = cslab, base = tb + cb, lim = 0xffffffffu;
-      } else if constexpr (TIPS == HX_TIPS_CODES) {
-        const int code = mycodes[(tile - t0) * kTile + nn];
-        bp = a.table, base = (uint32_t)(code * S + g * KV), lim = (uint32_t)(a.ncodes * S - 1);
-      } else {
-        bp = a.dense + (size_t)chh * a.N * S;
-        base = (uint32_t)(min(nbase + nn, a.N - 1) * S + g * KV), lim = (uint32_t)(a.N * S - 1);
-      }
-#pragma unroll
-      for (int q = 0; q < KV; ++q) arow[q] = bp[min(base + q, lim)] * ((nok && g * KV + q < S) ? R(1) : R(0));
-    }
+    // the parent's adjoint, then the weights (the child rows were issued by the
+    // previous tile)
     bool rok[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) rok[r] = nbase + acc_row<R>(g, r) < a.N;
@@ -874,6 +892,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
     for (int t = 0; t < MT; ++t)
 #pragma unroll
       for (int r = 0; r < 4; ++r) mine[acc_row<R>(g, r) * SP + 16 * t + nn] = acc[t][r];
+    // the next tile's child rows (the last tile re-reads its own, unused)
+    pre_rows<R, KV, TIPS>(a, internal, cslab, chh, mycodes, min(tile + 1, t1 - 1), t0, nn, g, S, CS,
+                          cb, arow);
     lds_barrier();
     // Ut_h = U_p * (P_other L_other), in the accumulator layout
     R u[MT][4];
@@ -927,7 +948,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
         for (int ks = 0; ks < KV; ++ks) {
           const R av = mine[nn * SP + g * KV + ks];  // K index (ks, g) = state g KV + ks
 #pragma unroll
-          for (int t = 0; t < MT; ++t) nu[t] = mma(av, Pt[t][ks], nu[t]);
+          for (int t = 0; t < MT; ++t) nu[t] = mma(av, ptw[(g * KV + ks) * kPtS + 16 * t + nn], nu[t]);
         }
       } else {
 #pragma unroll
@@ -1006,8 +1027,11 @@ static size_t mfma_lds(int C, int S, bool pre, int tiles_per_chunk) {
   const size_t SP = 16 * MT, rb = sizeof(R);
   // k_pre_mfma_r (MT <= 2) pads the per-row category entries to 4 and stages
   // two children's codes for the chunk
-  if (pre && (MT <= 2 || rb == 4))
-    return (4 * C * kTile * SP + 12 * kTile) * rb + 64 + 2 * (size_t)tiles_per_chunk * kTile;
+  if (pre && (MT <= 2 || rb == 4)) {
+    const int KV = MT == 2 && S <= 20 ? 5 : 4 * MT;
+    const size_t pt = MT <= 2 ? 2 * (size_t)C * 4 * KV * (SP + 1) : 0;  // transpose P tiles
+    return (4 * C * kTile * SP + 12 * kTile + pt) * rb + 64 + 2 * (size_t)tiles_per_chunk * kTile;
+  }
   if (pre) return (4 * C * kTile * SP + 3 * C * kTile) * rb + 64;
   return (C * kTile * SP + C * kTile) * rb + (size_t)(C * kTile + C * SP) * 8 + 3 * kTile * 4 + 64;
 }
