@@ -162,6 +162,8 @@ __device__ __forceinline__ void child_row(const PassArgs<R>& a, int ch, int k, i
 }  // namespace
 
 // ---------------------------------------------------------------- post-order
+// Launched for the root (ROOT = true: the per-chunk root sums of loglik, d/dfreqs
+// and d/dprops); the levels below use k_post_mfma_r.
 template <typename R, int MT, int KV, int TIPS, bool ROOT>
 __global__ __launch_bounds__(512) void k_post_mfma(PassArgs<R> a,
                                                    const int32_t* __restrict__ nodes, int S) {
@@ -1064,14 +1066,10 @@ int launch_mfma(bool post, const PassArgs<R>& a, const int32_t* nodes, int width
     if (root) HX_MF(KER, MTV, KVV, HX_TIPS_DENSE, true);                                     \
     else HX_MF(KER, MTV, KVV, HX_TIPS_DENSE, false);                                         \
   }
-#define HX_MF_KER(KER)                                                                       \
-  if (MT == 1) {                                                                             \
-    HX_MF_ALL(KER, 1, 4)                                                                     \
-  } else if (MT == 2) {                                                                      \
-    if (S <= 20) { HX_MF_ALL(KER, 2, 5) } else { HX_MF_ALL(KER, 2, 8) }                      \
-  } else {                                                                                   \
-    HX_MF_ALL(KER, 4, 16)                                                                    \
-  }
+// the root launch of the post-order (k_post_mfma_r serves every other level)
+#define HX_MF_ROOT(MTV, KVV)                                                                 \
+  if (tips == HX_TIPS_CODES) HX_MF(k_post_mfma, MTV, KVV, HX_TIPS_CODES, true);              \
+  else HX_MF(k_post_mfma, MTV, KVV, HX_TIPS_DENSE, true);
   if (post && !root) {
     // scale tiles, per-row maxima, the chunk's codes
     const size_t lds_r = (size_t)a.C * kTile * 16 * MT * sizeof(R) + 4 * kTile * sizeof(R) +
@@ -1094,7 +1092,13 @@ int launch_mfma(bool post, const PassArgs<R>& a, const int32_t* nodes, int width
     }
 #undef HX_MFR
   } else if (post) {
-    HX_MF_KER(k_post_mfma)
+    if (MT == 1) {
+      HX_MF_ROOT(1, 4)
+    } else if (MT == 2) {
+      if (S <= 20) { HX_MF_ROOT(2, 5) } else { HX_MF_ROOT(2, 8) }
+    } else {
+      HX_MF_ROOT(4, 16)
+    }
   } else if (MT == 1) {
     HX_MF_ALL(k_pre_mfma_r, 1, 4)
   } else if (MT == 2) {
@@ -1105,7 +1109,7 @@ int launch_mfma(bool post, const PassArgs<R>& a, const int32_t* nodes, int width
   } else {
     HX_MF_ALL(k_pre_mfma, 4, 16)
   }
-#undef HX_MF_KER
+#undef HX_MF_ROOT
 #undef HX_MF_ALL
 #undef HX_MF
   HX_HIP_TRY(hipGetLastError());
