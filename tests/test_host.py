@@ -217,3 +217,15 @@ def test_slice_for_fits_the_stream_workspace():
             assert lib.hx_job_workspace_bytes_for(T, S, sl, C, 32, code, 1) <= srv.ws_bytes
             if frac == 1.0:
                 assert sl == N
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; asked for more it exits with status 2 and says so, before
+    touching a device."""
+    import subprocess
+    import sys
+
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
