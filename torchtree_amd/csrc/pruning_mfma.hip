@@ -898,20 +898,26 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
     pre_rows<R, KV, TIPS>(a, internal, cslab, chh, mycodes, min(tile + 1, t1 - 1), t0, nn, g, S, CS,
                           cb, arow);
     lds_barrier();
-    // Ut_h = U_p * (P_other L_other), in the accumulator layout
+    // Ut_h = U_p * (P_other L_other), in the accumulator layout; the row maxima
+    // of Ut over (category, state) set the adjoint's rescale exponent
     R u[MT][4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int m = acc_row<R>(g, r);
-      R d = R(0);
+      R d = R(0), um = R(0);
 #pragma unroll
       for (int t = 0; t < MT; ++t) {
         u[t][r] = up[t][r] * other[m * SP + 16 * t + nn];
         d = fma(u[t][r], acc[t][r], d);
+        um = fmax_(um, u[t][r]);
       }
       if (h == 0) {
         d = rsum16(d);
         if (nn == 0) dens[m * 4 + c] = pc * d;
+      }
+      if (internal) {
+        um = rmax16(um);
+        if (nn == 0) mxx[(h * kTile + m) * 4 + c] = um;
       }
     }
     lds_barrier();
@@ -962,16 +968,11 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
             for (int e = 0; e < 4; ++e) nu[t] = mma(mine[nn * SP + g * KV + q + e], pv[e], nu[t]);
           }
       }
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        R m_ = R(0);
-#pragma unroll
-        for (int t = 0; t < MT; ++t) m_ = fmax_(m_, nu[t][r]);
-        m_ = rmax16(m_);
-        if (nn == 0) mxx[(h * kTile + acc_row<R>(g, r)) * 4 + c] = m_;
-      }
     }
-    lds_barrier();
+    // no barrier before the stores: the exponent comes from the Ut maxima shared
+    // at the previous barrier. U = P^T Ut is at most S times the largest Ut, so
+    // U 2^-e stays below 2 S; a power-of-two scale is exact and cancels in every
+    // term of G and of the likelihood ratios below this node
     if (internal) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -986,8 +987,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
       }
     }
     // no barrier: the next tile's first barrier orders this tile's reads of
-    // dens and mxx before their next writes, and each wave rewrites only its
-    // own rows of Yx and Lx before it
+    // dens and mxx before their next writes; each wave rewrites only its own
+    // rows of Yx (read by the sibling wave before the second barrier) and Lx
   }
   if (a.mats_dt != nullptr) {
     const R* q = a.mats_dt + (((size_t)k * a.B + chh) * C + c) * S * S;
