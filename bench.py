@@ -117,7 +117,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the pattern-sharded multi-GPU design is described in DESIGN.md", file=sys.stderr)
+              "across GPUs the path would shard patterns, one slice and one all-reduce per step",
+              file=sys.stderr)
         sys.exit(2)
     import torch
 
