@@ -10,6 +10,8 @@ Follows torchtree 1.0.2:
 * ``gtr_q``                       -> ``GTR.q`` (``nucleotide.py:328-374``)
 * ``general_symmetric_q``         -> ``GeneralSymmetricSubstitutionModel.q`` (``general.py:170-184``)
 * ``mg94_masks`` / ``mg94_q``     -> ``MG94.__init__`` / ``MG94.q`` (``codon.py:33-77``)
+* ``general_nonsymmetric_q``      -> ``GeneralNonSymmetricSubstitutionModel.q`` (``general.py:253-268``),
+  ``p_t_expm``                    -> ``NonSymmetricSubstitutionModel.p_t`` (``abstract.py:88-94``)
 * ``weibull_rates_probs``         -> ``UnivariateDiscretizedSiteModel.update_rates`` +
   ``WeibullSiteModel.inverse_cdf`` (``site_model.py:173-195,237-247``)
 * ``invariant_rates_probs``       -> ``InvariantSiteModel.update_rates_probs`` (``site_model.py:90-100``)
@@ -22,6 +24,9 @@ GTR, the MG94 row checks (``:298-310``), the Weibull tables
 (``test/test_site_model.py``), ``scipy.linalg.expm`` on the same Q, and the
 CTMC identities (rows sum to 1, P(0)=I, P(s)P(t)=P(s+t), detailed balance).
 The gamma quantiles are pinned by ``scipy.special.gammaincinv`` and mean 1.
+The non-symmetric Q and its P are pinned by the two-state closed form, a
+hand-written three-state mapping, and the reduction to the symmetric model
+(and to ``p_t_eigh``) when both triangles share their rates.
 """
 from __future__ import annotations
 
@@ -109,6 +114,26 @@ def general_symmetric_q(rates: torch.Tensor, mapping: torch.Tensor, pi: torch.Te
     ex = rates[..., mapping]
     lead = torch.broadcast_shapes(ex.shape[:-1], pi.shape[:-1])
     return q_from_exchangeabilities(ex.expand(lead + ex.shape[-1:]), pi.expand(lead + pi.shape[-1:]))
+
+
+def general_nonsymmetric_q(rates: torch.Tensor, mapping: torch.Tensor, pi: torch.Tensor) -> torch.Tensor:
+    """R from ``rates[mapping]``: the first half fills the upper triangle row by row, the second
+    half the mirrored lower positions; Q = R diag(pi), rows summing to 0 (general.py:253-268)."""
+    s = pi.shape[-1]
+    iu = torch.triu_indices(s, s, 1)
+    half = mapping.shape[-1] // 2
+    lead = torch.broadcast_shapes(rates.shape[:-1], pi.shape[:-1])
+    r = torch.zeros(lead + (s, s), dtype=rates.dtype)
+    r[..., iu[0], iu[1]] = rates[..., mapping[:half]].expand(lead + (half,))
+    r[..., iu[1], iu[0]] = rates[..., mapping[half:]].expand(lead + (half,))
+    q = r * pi[..., None, :]
+    return q - torch.diag_embed(q.sum(-1))
+
+
+def p_t_expm(q_unnorm: torch.Tensor, pi: torch.Tensor, tau: torch.Tensor) -> torch.Tensor:
+    """P(tau) = expm(Q tau) of the normalised Q (abstract.py:88-94); tau [..., B, C]."""
+    q = normalize_q(q_unnorm, pi)
+    return torch.matrix_exp(q[..., None, None, :, :] * tau[..., None, None])
 
 
 UNIVERSAL = "KNKNTTTTRSRSIIMIQHQHPPPPRRRRLLLLEDEDAAAAGGGGVVVV*Y*YSSSS*CWCLFLF"

@@ -103,6 +103,37 @@ def test_dropin_wide_stale_workspace(gpu, S, C, T, dtype, rl, rg, monkeypatch):
         assert err < rg, (name, err)
 
 
+@pytest.mark.parametrize("S,C", [(4, 4), (20, 2)])
+@pytest.mark.parametrize("dtype,rl,rg", [(F64, 1e-12, 1e-10), (torch.float32, 1e-6, 1e-5)])
+def test_dropin_nonreversible(gpu, S, C, dtype, rl, rg):
+    """Pruning with non-reversible P = expm(Q t) (abstract.py:88-94): the kernels assume no
+    symmetry of P; loglik and d/dP against the oracle."""
+    print(f"case: non-reversible drop-in S={S} C={C} {dtype}", flush=True)
+    from torchtree_amd import likelihood as hl
+
+    T, N, K = 12, 150, 2
+    post, tips, codes, w, pi, ex, mats, rates, props, bl = _random_case(S, T, N, K, C, 5 * S + C)
+    g = torch.Generator().manual_seed(S + C)
+    nonsym = torch.rand(S * (S - 1), generator=g, dtype=F64) * 2 + 0.05
+    q = omodels.general_nonsymmetric_q(nonsym, torch.randperm(S * (S - 1), generator=g), pi)
+    mats = omodels.p_t_expm(q, pi, bl.unsqueeze(-1) * rates.reshape(1, 1, -1))
+    m = mats.detach().clone().requires_grad_(True)
+    f = pi.reshape(1, -1).clone().requires_grad_(True)
+    p = props.reshape(-1, 1, 1).clone().requires_grad_(True)
+    ref = opruning.treelik_discrete_rescaled(tips + [None] * (T - 1), w, post, m, f, p)
+    ref.sum().backward()
+    m2 = mats.detach().to(gpu, dtype).requires_grad_(True)
+    f2 = pi.reshape(1, -1).to(gpu, dtype).requires_grad_(True)
+    p2 = props.reshape(-1, 1, 1).to(gpu, dtype).requires_grad_(True)
+    got = hl.calculate_treelikelihood_discrete(tips + [None] * (T - 1), w, post, m2, f2, p2)
+    rel = float(((got.detach().double().cpu() - ref.detach()).abs() / ref.detach().abs()).max())
+    assert rel < rl, rel
+    got.sum().backward()
+    for a, b, name in ((m2.grad, m.grad, "mats"), (f2.grad, f.grad, "freqs"), (p2.grad, p.grad, "props")):
+        err = float((a.double().cpu() - b).norm() / b.norm())
+        assert err < rg, (name, err)
+
+
 @pytest.mark.parametrize("dtype,rl,rg", [(F64, 1e-12, 1e-9), (torch.float32, 1e-6, 1e-5)])
 def test_engine_aa(gpu, dtype, rl, rg):
     print(f"case: engine AA S=20 gamma4 {dtype}", flush=True)

@@ -159,3 +159,47 @@ def test_gamma_quantiles():
     raw = gdist.ppf(q, 0.5, scale=1 / 0.5)
     assert np.allclose(r.numpy(), raw / raw.mean(), rtol=1e-12)
     assert abs(float((r * p).sum()) - 1.0) < 1e-14
+
+
+def test_nonsymmetric_two_state_closed_form():
+    """Two states: Q01 = a = r01 pi1, Q10 = b = r10 pi0, normalised by pi0 a + pi1 b; then
+    P(t) = [[b + a e, a (1 - e)], [b (1 - e), a + b e]] / (a + b) with e = exp(-(a + b) t)."""
+    rates = torch.tensor([0.7, 2.3], dtype=F64)
+    pi = torch.tensor([0.35, 0.65], dtype=F64)
+    mapping = torch.tensor([0, 1])
+    q = models.general_nonsymmetric_q(rates, mapping, pi)
+    a, b = 0.7 * 0.65, 2.3 * 0.35
+    assert torch.allclose(q, torch.tensor([[-a, a], [b, -b]], dtype=F64), atol=1e-15)
+    norm = 0.35 * a + 0.65 * b
+    a, b = a / norm, b / norm
+    for t in (0.01, 0.3, 2.0):
+        e = np.exp(-(a + b) * t)
+        want = torch.tensor([[b + a * e, a * (1 - e)], [b * (1 - e), a + b * e]], dtype=F64) / (a + b)
+        got = models.p_t_expm(q, pi, torch.tensor([[t]], dtype=F64))[0, 0]
+        assert torch.allclose(got, want, atol=1e-14)
+
+
+def test_nonsymmetric_three_state_mapping():
+    """The first half of the mapping fills (0,1), (0,2), (1,2); the second (1,0), (2,0), (2,1)."""
+    rates = torch.tensor([1.0, 2.0, 3.0, 4.0, 5.0, 6.0], dtype=F64)
+    pi = torch.tensor([0.2, 0.3, 0.5], dtype=F64)
+    q = models.general_nonsymmetric_q(rates, torch.tensor([5, 4, 3, 2, 1, 0]), pi)
+    r = torch.tensor([[0.0, 6.0, 5.0], [3.0, 0.0, 4.0], [2.0, 1.0, 0.0]], dtype=F64)
+    want = r * pi
+    want = want - torch.diag(want.sum(-1))
+    assert torch.allclose(q, want, atol=1e-15)
+
+
+def test_nonsymmetric_reduces_to_symmetric():
+    """Equal rates in both triangles give the symmetric model's Q, and expm gives p_t_eigh."""
+    g = torch.Generator().manual_seed(4)
+    s, d = 5, 10
+    rates = torch.rand(d, generator=g, dtype=F64) + 0.1
+    pi = torch.rand(s, generator=g, dtype=F64) + 0.2
+    pi = pi / pi.sum()
+    mapping = torch.cat((torch.arange(d), torch.arange(d)))
+    qn = models.general_nonsymmetric_q(rates, mapping, pi)
+    qs = models.general_symmetric_q(rates, torch.arange(d), pi)
+    assert torch.allclose(qn, qs, atol=1e-15)
+    tau = torch.tensor([[0.05, 0.4], [1.1, 0.0]], dtype=F64)
+    assert torch.allclose(models.p_t_expm(qn, pi, tau), models.p_t_eigh(qs, pi, tau), atol=1e-12)

@@ -68,6 +68,29 @@ def general_symmetric_q(rates, mapping, pi):
     return q_from_exchangeabilities(rates[..., mapping], pi)
 
 
+def general_nonsymmetric_q(rates, mapping, pi):
+    """Non-reversible Q (general.py:253-268): ``rates[mapping]`` with the first half of the
+    mapping on the upper triangle (row by row) and the second half on the mirrored lower
+    positions; Q = R diag(pi) with rows summing to zero."""
+    s = pi.shape[-1]
+    iu = torch.triu_indices(s, s, 1, device=rates.device)
+    half = mapping.shape[-1] // 2
+    lead = torch.broadcast_shapes(rates.shape[:-1], pi.shape[:-1])
+    upper = rates[..., mapping[:half]].expand(lead + (half,))
+    lower = rates[..., mapping[half:]].expand(lead + (half,))
+    r = torch.zeros(lead + (s, s), dtype=rates.dtype, device=rates.device)
+    r[..., iu[0], iu[1]] = upper
+    r[..., iu[1], iu[0]] = lower
+    q = r * pi[..., None, :]
+    return q - torch.diag_embed(q.sum(-1))
+
+
+def p_t_expm(q_norm, tau):
+    """P = expm(Q tau) for a normalised non-reversible Q [..., S, S] and tau [..., B, C]
+    (abstract.py:88-94), in the dtype of Q; differentiable through torch."""
+    return torch.matrix_exp(q_norm[..., None, None, :, :] * tau[..., None, None])
+
+
 UNIVERSAL_CODE = "KNKNTTTTRSRSIIMIQHQHPPPPRRRRLLLLEDEDAAAAGGGGVVVV*Y*YSSSS*CWCLFLF"
 
 

@@ -36,7 +36,7 @@ import torch
 from . import alignment as al
 from . import models as hm
 from .engine import TreeJob
-from .likelihood import TipData
+from .likelihood import TipData, _PruneFn
 from .tree import (Topology, heights_to_ratios, leaf_heights_from_dates, ratio_bounds,
                    ratios_to_heights, time_branch_lengths)
 
@@ -308,6 +308,8 @@ def _reparam_tree(data, dic):
 class SubstModel:
     """Normalised Q [..., S, S] and frequencies [..., S] read at call time."""
 
+    nonsymmetric = False  # P through expm (non-reversible Q) instead of the eigensystem
+
     def __init__(self, S, qfn, ffn, shape_fn):
         self.state_count = S
         self._q = qfn
@@ -376,6 +378,25 @@ def _general(data, dic):
         mapping = process_object(data["mapping"], dic).tensor.long()
     return SubstModel(S, lambda: hm.general_symmetric_q(rates.tensor, mapping, freqs.tensor),
                       lambda: freqs.tensor, lambda: _shape_of(rates, freqs))
+
+
+@register("GeneralNonSymmetricSubstitutionModel")
+def _general_nonsymmetric(data, dic):
+    """Non-reversible model (general.py:202-291); P = expm(Q t) as in abstract.py:88-94."""
+    dt = _data_type(data["data_type"], dic)
+    rates = process_object(data["rates"], dic)
+    freqs = process_object(data["frequencies"], dic)
+    S = dt.state_count
+    if "mapping" not in data:
+        mapping = torch.arange(S * (S - 1))
+    elif isinstance(data["mapping"], list):
+        mapping = torch.tensor(data["mapping"])
+    else:
+        mapping = process_object(data["mapping"], dic).tensor.long()
+    model = SubstModel(S, lambda: hm.general_nonsymmetric_q(rates.tensor, mapping, freqs.tensor),
+                       lambda: freqs.tensor, lambda: _shape_of(rates, freqs))
+    model.nonsymmetric = True
+    return model
 
 
 @register("EmpiricalSubstitutionModel")
@@ -544,12 +565,29 @@ class TreeLikelihoodModel:
         probs = probs.reshape(-1, probs.shape[-1]).expand(K, -1) if probs.dim() > 1 else probs.unsqueeze(0)
         q = self.subst_model.q_norm()
         f = self.subst_model.frequencies()
+        if self.subst_model.nonsymmetric:
+            ll = self._loglik_expm(bls, rates, probs, q, f)
+            return ll.to(torch.get_default_dtype()).cpu().reshape(shape + (1,))
         if q.dim() == 3 and q.shape[0] == 1:
             q, f = q[0], f.reshape(-1, f.shape[-1])[0]
         ll = self.job.loglik(bls.to(torch.float64), rates.to(torch.float64),
                              probs.to(torch.float64), q.to(torch.float64), f.to(torch.float64))
         out = ll.to(torch.get_default_dtype()).cpu()
         return out.reshape(shape + (1,))
+
+    def _loglik_expm(self, bls, rates, probs, q, f):
+        """Non-reversible Q: P = expm(Q bl r) by torch.matrix_exp in FP64 on the device
+        (abstract.py:88-94), then the pruning kernels and their d/dP through the drop-in
+        autograd function; the gradient reaches Q through matrix_exp's backward."""
+        dev = self.device
+        q = q.to(dev, torch.float64)
+        if q.dim() == 2:
+            q = q.unsqueeze(0)
+        tau = bls.to(dev, torch.float64)[:, :, None] * rates.to(dev, torch.float64)[:, None, :]
+        mats = hm.p_t_expm(q, tau).to(self.dtype).contiguous()  # [K, B, C, S, S]
+        f2 = f.to(dev, self.dtype).reshape(-1, f.shape[-1]).contiguous()
+        p2 = probs.to(dev, self.dtype).contiguous()
+        return _PruneFn.apply(mats, f2, p2, self.job.pruning)
 
     @classmethod
     def from_json(cls, data, dic):
