@@ -233,3 +233,65 @@ def test_json_nonsymmetric_model(gpu, tiny_newick, tiny_fasta):
             assert abs(fd - float(g[i])) < 1e-6 * max(1.0, abs(fd)), (i, fd, float(g[i]))
     finally:
         torch.set_default_dtype(prev_dtype)
+
+
+@pytest.mark.parametrize("kind", ["GeneralSymmetricSubstitutionModel", "GeneralNonSymmetricSubstitutionModel"])
+def test_json_attribute_pattern(gpu, tiny_newick, tiny_fasta, kind):
+    """Discrete trait (AttributePattern, attribute_pattern.py:13-74): one pattern, each taxon's
+    attribute as its state, under a 3-state general model; against the oracle on the same tree,
+    branch lengths and P."""
+    print(f"case: JSON attribute pattern {kind}", flush=True)
+    from oracle import models as omodels
+    from oracle import pruning as opruning
+    from torchtree_amd.treelikelihood_model import TreeLikelihoodModel
+
+    taxa = _taxa(tiny_fasta)
+    regions = ["east", "west", "north"]
+    for i, t in enumerate(taxa["taxa"]):
+        t["attributes"] = {"region": regions[(i * 2 + 1) % 3]}
+    sym = kind.startswith("GeneralSymmetric")
+    rates = [0.7, 1.9, 0.4] if sym else [0.7, 1.9, 0.4, 1.2, 0.3, 2.6]
+    dtype_spec = {"id": "regions", "type": "GeneralDataType", "codes": regions}
+    data = {
+        "id": "like", "type": "TreeLikelihoodModel", "precision": "float64",
+        "tree_model": {"id": "tree", "type": "UnRootedTreeModel", "newick": tiny_newick,
+                       "branch_lengths": {"id": "branches", "type": "Parameter", "tensor": [0.0],
+                                          "dtype": "torch.float64"},
+                       "keep_branch_lengths": True, "taxa": taxa},
+        "site_model": {"id": "sm", "type": "ConstantSiteModel"},
+        "substitution_model": {"id": "q", "type": kind, "data_type": dtype_spec,
+                               "rates": {"id": "rates", "type": "Parameter", "tensor": rates,
+                                         "dtype": "torch.float64"},
+                               "frequencies": {"id": "freqs", "type": "Parameter",
+                                               "tensor": [0.5, 0.3, 0.2], "dtype": "torch.float64"}},
+        "site_pattern": {"id": "sp", "type": "AttributePattern", "taxa": "taxa",
+                         "data_type": "regions", "attribute": "region"},
+    }
+    prev_dtype = torch.get_default_dtype()
+    torch.set_default_dtype(torch.float64)
+    try:
+        like = TreeLikelihoodModel.from_json(data, {})
+        got = float(like())
+        topo = like.tree_model.topo
+        bl = like.tree_model.branch_lengths().detach().reshape(-1)
+        bls = torch.cat((bl, torch.zeros(1, dtype=torch.float64)))
+        pi = torch.tensor([0.5, 0.3, 0.2], dtype=torch.float64)
+        r = torch.tensor(rates, dtype=torch.float64)
+        if sym:
+            q = omodels.general_symmetric_q(r, torch.arange(3), pi)
+            mats = omodels.p_t_eigh(q, pi, bls[None, :, None])
+        else:
+            q = omodels.general_nonsymmetric_q(r, torch.arange(6), pi)
+            mats = omodels.p_t_expm(q, pi, bls[None, :, None])
+        tips = []
+        for t in taxa["taxa"]:
+            s = regions.index(t["attributes"]["region"])
+            tips.append(torch.nn.functional.one_hot(torch.tensor([s]), 3).to(torch.float64).t())
+        post = [tuple(int(x) for x in row) for row in topo.postorder]
+        ref = opruning.treelik_discrete_rescaled(tips + [None] * (len(tips) - 1),
+                                                 torch.ones(1, dtype=torch.float64), post, mats,
+                                                 pi.reshape(1, -1),
+                                                 torch.ones(1, 1, 1, dtype=torch.float64))
+        assert abs(got - float(ref)) < 1e-10 * abs(float(ref)), (got, float(ref))
+    finally:
+        torch.set_default_dtype(prev_dtype)

@@ -199,6 +199,38 @@ class SitePattern:
         return al.compress_bytes(seqs, self.alignment.data_type.size)
 
 
+class AttributePattern:
+    """Discrete-trait pattern (attribute_pattern.py:13-74): one pattern of weight 1 whose
+    state at each taxon is that taxon's ``attribute`` value under ``data_type``."""
+
+    def __init__(self, taxa, data_type, attribute):
+        self.taxa, self.data_type, self.attribute = taxa, data_type, attribute
+
+    def tip_codes(self, use_ambiguities=False, use_tip_states=False):
+        """uint8 codes [T][1], the table of their partial rows, weights [1].
+
+        Tip states (compute_tips_states) are one-hot, with states past the last one clamped to
+        the all-ones row; otherwise each value's ``partial`` row (compute_tips_partials)."""
+        S = self.data_type.state_count
+        rows = []
+        for t in self.taxa:
+            v = t[self.attribute]
+            if use_tip_states:
+                st = min(self.data_type.state(v), S)
+                rows.append(tuple(1.0 if i == st else 0.0 for i in range(S)) if st < S else (1.0,) * S)
+            else:
+                rows.append(tuple(self.data_type.partial(v, use_ambiguities)))
+        uniq = sorted(set(rows))
+        codes = torch.tensor([[uniq.index(r)] for r in rows], dtype=torch.uint8)
+        return codes, torch.tensor(uniq, dtype=torch.float64), torch.ones(1, dtype=torch.float64)
+
+
+@register("AttributePattern")
+def _attribute_pattern(data, dic):
+    taxa = process_object(data["taxa"], dic)
+    return AttributePattern(taxa, _data_type(data["data_type"], dic), data["attribute"])
+
+
 @register("SitePattern")
 def _site_pattern(data, dic):
     if data.get("indices"):
@@ -530,9 +562,13 @@ class TreeLikelihoodModel:
         if precision is None:
             precision = torch.get_default_dtype()
         self.dtype = _DTYPES[precision] if isinstance(precision, str) else precision
-        cols, weights = site_pattern.compressed()
-        dt = site_pattern.alignment.data_type
-        codes, table = al.encode_tips(cols, dt, use_ambiguities, use_tip_states)
+        if isinstance(site_pattern, AttributePattern):
+            dt = site_pattern.data_type
+            codes, table, weights = site_pattern.tip_codes(use_ambiguities, use_tip_states)
+        else:
+            cols, weights = site_pattern.compressed()
+            dt = site_pattern.alignment.data_type
+            codes, table = al.encode_tips(cols, dt, use_ambiguities, use_tip_states)
         S = dt.state_count
         topo = tree_model.topo
         tips = TipData(0, codes.contiguous().to(self.device), table.to(self.device), S,
