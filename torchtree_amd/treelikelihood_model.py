@@ -12,12 +12,14 @@ every parameter tensor (``requires_grad``):
 * ``Parameter`` (``core/parameter.py:189``): ``tensor``, ``full``, ``zeros``, ``ones``
   (+ ``value``, ``dtype``, ``requires_grad``).
 * ``Taxa``/``Taxon``, ``Alignment`` (nucleotide / amino acid / codon / general data
-  types; ``sequences`` or ``file``), ``SitePattern``.
+  types; ``sequences`` or ``file``), ``SitePattern``, ``AttributePattern``
+  (``attribute_pattern.py:13-74``, discrete traits).
 * ``UnRootedTreeModel`` (``tree_model.py:225-344``), ``TimeTreeModel`` (349-463),
   ``ReparameterizedTimeTreeModel`` with ``ratios``/``root_height`` (527-707).
 * ``JC69``, ``HKY``, ``GTR`` (``nucleotide.py``), ``GeneralJC69``,
-  ``GeneralSymmetricSubstitutionModel``, ``EmpiricalSubstitutionModel``
-  (``general.py``), ``MG94`` (``codon.py``).
+  ``GeneralSymmetricSubstitutionModel``, ``EmpiricalSubstitutionModel``,
+  ``GeneralNonSymmetricSubstitutionModel`` (``general.py``; P = expm(Q t) through
+  torch, as ``abstract.py:88-94``), ``MG94`` (``codon.py``).
 * ``ConstantSiteModel``, ``InvariantSiteModel``, ``WeibullSiteModel``
   (``site_model.py``) and ``GammaSiteModel`` (new; torchtree has none).
 * ``StrictClockModel``, ``SimpleClockModel`` (``branch_model.py:38-77``).
