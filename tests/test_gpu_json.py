@@ -270,13 +270,17 @@ def test_json_attribute_pattern(gpu, tiny_newick, tiny_fasta, kind):
     prev_dtype = torch.get_default_dtype()
     torch.set_default_dtype(torch.float64)
     try:
-        like = TreeLikelihoodModel.from_json(data, {})
-        got = float(like())
+        dic = {}
+        like = TreeLikelihoodModel.from_json(data, dic)
+        dic["rates"].tensor.requires_grad_(True)
+        out = like()
+        out.sum().backward()
+        got, grad = float(out), dic["rates"].tensor.grad.clone()
         topo = like.tree_model.topo
         bl = like.tree_model.branch_lengths().detach().reshape(-1)
         bls = torch.cat((bl, torch.zeros(1, dtype=torch.float64)))
         pi = torch.tensor([0.5, 0.3, 0.2], dtype=torch.float64)
-        r = torch.tensor(rates, dtype=torch.float64)
+        r = torch.tensor(rates, dtype=torch.float64, requires_grad=True)
         if sym:
             q = omodels.general_symmetric_q(r, torch.arange(3), pi)
             mats = omodels.p_t_eigh(q, pi, bls[None, :, None])
@@ -293,5 +297,9 @@ def test_json_attribute_pattern(gpu, tiny_newick, tiny_fasta, kind):
                                                  pi.reshape(1, -1),
                                                  torch.ones(1, 1, 1, dtype=torch.float64))
         assert abs(got - float(ref)) < 1e-10 * abs(float(ref)), (got, float(ref))
+        # one pattern (N = 1): the gradient pass reads past the only row of every slab
+        ref.sum().backward()
+        err = float((grad - r.grad).norm() / r.grad.norm())
+        assert err < 1e-9, (err, grad, r.grad)
     finally:
         torch.set_default_dtype(prev_dtype)
