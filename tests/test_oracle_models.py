@@ -6,7 +6,9 @@ batched form (test/test_substitution_model.py:53-136), the HKY tables
 MG94 rows (:298-310), Weibull rate tables (test/test_site_model.py:26-137).
 Independent checks: scipy.linalg.expm of the same Q, CTMC identities
 (rows sum to 1, P(0)=I, P(s)P(t)=P(s+t), detailed balance, pi P = pi) and
-scipy.special.gammaincinv for the gamma quantiles.
+scipy.special.gammaincinv for the gamma quantiles. The non-symmetric Q and its
+expm are pinned by the two-state closed form, a hand-written three-state
+mapping and the reduction to the symmetric model (general.py:202-291).
 """
 import numpy as np
 import pytest
