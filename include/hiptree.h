@@ -15,7 +15,10 @@
  *   props       R [K or 1][C]
  *   weights     double [N]
  *   tip codes   uint8 [T][N] with a table R [ncodes][S] of tip partial rows,
- *               or dense tip partials R [T][N][S]
+ *               or dense tip partials R [T][N][S]. Every code must be below
+ *               ncodes: the kernels index the table with the raw code
+ *               (hx_job_create checks this on the host; device-resident codes
+ *               passed to hx_loglik* are the caller's to keep in range).
  *
  * Reference interfaces replaced (torchtree 1.0.2, torchtree/evolution/):
  *   hx_loglik      -> calculate_treelikelihood (tree_likelihood.py:14-37),

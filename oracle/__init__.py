@@ -14,7 +14,10 @@ pruning   the six ``calculate_treelikelihood*`` functions
 models    Q builders, P(t), site models (rates / proportions)
 data      data types, FASTA, pattern compression, Newick + node indexing,
           time-tree branch lengths
-grad      autograd gradients and central finite differences
+
+Gradients of the oracle are torch autograd through these functions (the
+reference's own gradient mechanism) and central finite differences, both
+taken in the tests.
 
 How each function is pinned (golden values, brute force, closed forms,
 library cross-checks) is recorded in the module docstrings and in

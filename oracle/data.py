@@ -345,11 +345,15 @@ def codon_partial(triplet: str, table: str, state_count: int):
 
 # --- pattern compression ----------------------------------------------------
 
-def compress(sequences: list[str], size: int = 1):
+def compress(sequences: list[str], size: int = 1, indices=None):
     """Unique columns sorted lexicographically with integer counts (site_pattern.py:69-97).
 
-    ``sequences`` are in taxa order; returns (patterns per taxon, weights).
+    ``sequences`` are in taxa order; returns (patterns per taxon, weights). ``indices`` (ints
+    and slices, one-character data only) first replaces each sequence by the concatenation of
+    ``sequence[index]`` over the list, as the reference does.
     """
+    if indices is not None:
+        sequences = ["".join(s[i] for i in indices) for s in sequences]
     if size > 1:
         cols = list(zip(*[[s[i:i + size] for i in range(0, len(s), size)] for s in sequences]))
     else:

@@ -156,19 +156,6 @@ def treelik_tip_states_rescaled(partials, weights, post_indexing, mats, freqs, p
     return ((site + log_scale) * weights).sum(-1)
 
 
-def site_loglik_discrete(partials, post_indexing, mats, freqs, props):
-    """Per-pattern log-likelihood [..., 1, N] with rescaling (no weights)."""
-    part = list(partials)
-    log_scale = 0.0
-    for node, left, right in post_indexing:
-        p = _child_term(mats[..., left, :, :, :], part[left]) * _child_term(
-            mats[..., right, :, :, :], part[right]
-        )
-        part[node], scaler = _scale_node(p)
-        log_scale = log_scale + torch.log(scaler)
-    return torch.log(_root_site_lik(part[post_indexing[-1][0]], freqs, props)) + log_scale
-
-
 def brute_force_loglik(tips, weights, post_indexing, mats, freqs, props):
     """Sum over every assignment of internal states (small trees only).
 

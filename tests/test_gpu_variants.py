@@ -1,0 +1,204 @@
+"""The drop-in variants that take tip states or a rescaling threshold, on the GPU.
+
+* ``calculate_treelikelihood_tip_states_discrete`` and ``..._rescaled``
+  (``tree_likelihood.py:78-131,224-278``) against the oracle's ``treelik_tip_states(_rescaled)``,
+  with tip codes made by ``compress_alignment_states`` (``site_pattern.py:127-151``: every
+  ambiguous or unknown character clamped to S, which selects the all-ones column);
+* ``calculate_treelikelihood_discrete_safe`` (``tree_likelihood.py:134-183``) against the
+  oracle's ``treelik_discrete_safe`` in the regime it exists for: internal partials that fall
+  below the threshold (1e-40 in FP64, 1e-20 in FP32, ``tree_likelihood.py:302-304``), where the
+  unscaled FP32 pass returns -inf;
+* JSON ``use_tip_states: true`` on the reference's tiny model (``test_tree_likelihood.py:73-169``,
+  -83.329016);
+* a branch so long that exp(lambda t) underflows: the branch-length gradient stays finite and
+  matches the oracle.
+
+Tolerances: FP64 relative 1e-12 (logL) and 1e-10 (gradients); FP32 1e-6 and 1e-5 (norm-wise).
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import data as odata
+from oracle import models as omodels
+from oracle import pruning as opruning
+
+pytestmark = pytest.mark.gpu
+F64 = torch.float64
+
+
+def _flush(msg):
+    print(msg, flush=True)
+
+
+def _random_alignment(kind, T, L, rng):
+    if kind == "nucleotide":
+        chars = np.array(list("ACGT") * 8 + list("RYNKM-?acgt"))
+    else:
+        chars = np.array(list("ACDEFGHIKLMNPQRSTVWY") * 3 + list("BZX-?*acd"))
+    return ["".join(rng.choice(chars, L)) for _ in range(T)]
+
+
+def _tree_and_mats(T, S, C, rng, dtype=F64):
+    from torchtree_amd import synthetic
+
+    topo = synthetic.random_topology(T, rng)
+    post = [tuple(int(x) for x in r) for r in topo.postorder]
+    pi = torch.tensor(rng.dirichlet(np.full(S, 5.0)), dtype=F64)
+    exch = torch.tensor(rng.gamma(2.0, 0.5, S * (S - 1) // 2), dtype=F64)
+    q = omodels.q_from_exchangeabilities(exch, pi)
+    tau = torch.tensor(rng.exponential(0.15, (2, 2 * T - 2, 1)), dtype=F64) * \
+        torch.linspace(0.4, 1.6, C, dtype=F64)
+    mats = omodels.p_t_eigh(q, pi.expand(2, S), tau)  # [K=2, B, C, S, S]
+    props = torch.tensor(rng.dirichlet(np.ones(C)), dtype=F64).reshape(-1, 1, 1)
+    return post, mats, pi.reshape(1, S), props
+
+
+@pytest.mark.parametrize("kind,S", [("nucleotide", 4), ("amino_acid", 20)])
+@pytest.mark.parametrize("dtype,rl,rg", [(F64, 1e-12, 1e-10), (torch.float32, 1e-6, 1e-5)])
+@pytest.mark.parametrize("rescaled", [False, True])
+def test_tip_states_shims(gpu, kind, S, dtype, rl, rg, rescaled):
+    _flush(f"case: tip-states shim {kind} {dtype} rescaled={rescaled}")
+    from torchtree_amd import likelihood as hl
+
+    rng = np.random.default_rng(11 + S)
+    T, C = 14, 3
+    seqs = _random_alignment(kind, T, 400, rng)
+    code_fn = odata.nucleotide_code if kind == "nucleotide" else odata.amino_acid_code
+    codes, weights = odata.compress_alignment_states(seqs, code_fn, S)
+    assert any(int((c == S).sum()) > 0 for c in codes)  # the all-ones column is exercised
+    post, mats, freqs, props = _tree_and_mats(T, S, C, rng)
+    m = mats.clone().requires_grad_(True)
+    f = freqs.clone().requires_grad_(True)
+    p = props.clone().requires_grad_(True)
+    ofn = opruning.treelik_tip_states_rescaled if rescaled else opruning.treelik_tip_states
+    ref = ofn(list(codes) + [None] * (T - 1), weights, post, m, f, p)
+    ref.sum().backward()
+    hfn = (hl.calculate_treelikelihood_tip_states_discrete_rescaled if rescaled
+           else hl.calculate_treelikelihood_tip_states_discrete)
+    md = mats.to(gpu, dtype).requires_grad_(True)
+    fd = freqs.to(gpu, dtype).requires_grad_(True)
+    pd = props.to(gpu, dtype).requires_grad_(True)
+    got = hfn([c.to(gpu) for c in codes] + [None] * (T - 1), weights.to(gpu), post, md, fd, pd)
+    assert got.shape == ref.shape
+    got.sum().backward()
+    rel = float(((got.detach().cpu().double() - ref.detach()).abs() / ref.detach().abs()).max())
+    assert rel < rl, rel
+    for name, a, b in (("mats", md.grad, m.grad), ("freqs", fd.grad, f.grad),
+                       ("props", pd.grad, p.grad)):
+        err = float((a.cpu().double() - b).norm() / b.norm())
+        assert err < rg, (name, err)
+
+
+def _deep_case(T, rng):
+    """Random tip states on a large tree: internal partials shrink like S^-(tips below)."""
+    from torchtree_amd import synthetic
+
+    topo = synthetic.random_topology(T, rng)
+    post = [tuple(int(x) for x in r) for r in topo.postorder]
+    N = 40
+    tbl = torch.eye(4, dtype=F64)
+    tips = [tbl[torch.tensor(rng.integers(0, 4, N))].t().contiguous() for _ in range(T)]
+    weights = torch.tensor(rng.integers(1, 4, N), dtype=F64)
+    pi = torch.tensor([0.3, 0.2, 0.2, 0.3], dtype=F64)
+    q = omodels.hky_q(torch.tensor([2.5], dtype=F64), pi)
+    tau = torch.tensor(rng.exponential(0.3, (2 * T - 2, 1)), dtype=F64) * \
+        torch.tensor([0.5, 1.5], dtype=F64)
+    mats = omodels.p_t_eigh(q, pi, tau)  # [B, C, S, S]
+    props = torch.tensor([0.5, 0.5], dtype=F64).reshape(-1, 1, 1)
+    return post, tips, weights, mats, pi.reshape(1, 4), props
+
+
+@pytest.mark.parametrize("dtype,threshold,rl,rg", [(F64, 1e-40, 1e-12, 1e-10),
+                                                   (torch.float32, 1e-20, 1e-6, 1e-5)])
+def test_safe_shim_below_threshold(gpu, dtype, threshold, rl, rg):
+    _flush(f"case: _safe shim {dtype}")
+    from torchtree_amd import likelihood as hl
+
+    rng = np.random.default_rng(23)
+    T = 160
+    post, tips, weights, mats, freqs, props = _deep_case(T, rng)
+    m = mats.clone().requires_grad_(True)
+    # the reference calls _safe after the unscaled pass filled the internal partials, inside
+    # the same autograd graph (tree_likelihood.py:378-388), so nodes it keeps carry gradient
+    filled = opruning.fill_internal(tips + [None] * (T - 1), post, m)
+    # the regime _safe is for: some internal partials below the threshold
+    assert min(float(filled[v].detach().max()) for v, _, _ in post) < threshold
+    ref = opruning.treelik_discrete_safe(filled, weights, post, m, freqs, props, threshold)
+    ref.sum().backward()
+    assert math.isfinite(float(ref))
+    if dtype == torch.float32:
+        # the plain FP32 pass underflows: this is where the reference switches to _safe
+        plain = opruning.treelik_discrete([t.float() for t in tips] + [None] * (T - 1),
+                                          weights.float(), post, mats.float(), freqs.float(),
+                                          props.float())
+        assert math.isinf(float(plain))
+    md = mats.to(gpu, dtype).requires_grad_(True)
+    got = hl.calculate_treelikelihood_discrete_safe(
+        [t.to(gpu, dtype) for t in tips] + [None] * (T - 1), weights.to(gpu), post, md,
+        freqs.to(gpu, dtype), props.to(gpu, dtype), threshold)
+    got.sum().backward()
+    rel = abs(float(got) - float(ref)) / abs(float(ref))
+    assert rel < rl, rel
+    err = float((md.grad.cpu().double() - m.grad).norm() / m.grad.norm())
+    assert err < rg, err
+
+
+@pytest.mark.parametrize("precision,tol", [("float64", 1e-9), ("float32", 1e-6)])
+def test_tiny_json_tip_states(gpu, tiny_newick, tiny_fasta, precision, tol):
+    """use_tip_states (tree_likelihood.py:437) on tiny: the reference's -83.329016."""
+    _flush(f"case: tiny JSON use_tip_states {precision}")
+    from tests.test_gpu_json import _taxa
+    from torchtree_amd.treelikelihood_model import TreeLikelihoodModel
+
+    data = {
+        "id": "like", "type": "TreeLikelihoodModel", "precision": precision,
+        "use_tip_states": True,
+        "tree_model": {"id": "tree", "type": "UnRootedTreeModel", "newick": tiny_newick,
+                       "branch_lengths": {"id": "branches", "type": "Parameter",
+                                          "tensor": [0.0], "dtype": "torch.float64"},
+                       "keep_branch_lengths": True, "taxa": _taxa(tiny_fasta)},
+        "site_model": {"id": "sm", "type": "ConstantSiteModel"},
+        "substitution_model": {"id": "m", "type": "JC69"},
+        "site_pattern": {"id": "sp", "type": "SitePattern",
+                         "alignment": {"id": "alignment", "type": "Alignment",
+                                       "datatype": "nucleotide", "file": tiny_fasta,
+                                       "taxa": "taxa"}},
+    }
+    like = TreeLikelihoodModel.from_json(data, {})
+    out = like()
+    assert out.shape == (1,)
+    assert abs(float(out) - (-83.329016)) < 83.33 * max(tol, 1e-7) + 1e-6
+
+
+def test_long_branch_gradient_finite(gpu):
+    """exp(lambda * t) underflows on a branch of length 1000: the divided differences of exp on
+    the spectrum must stay finite (a naive exp(b) * expm1(a - b) / (a - b) gives 0 * inf)."""
+    _flush("case: long-branch gradient")
+    from torchtree_amd import models as hm
+    from torchtree_amd import synthetic
+    from torchtree_amd.engine import TreeJob
+
+    wl = synthetic.make_workload(12, 200, 1, model="gtr", seed=4, bl_mean=0.1)
+    bls = wl.bls.clone()
+    bls[0, 3] = 1000.0
+    bls[0, 7] = 900.0
+    post = [tuple(int(x) for x in r) for r in wl.topo.postorder]
+    b = bls.clone().requires_grad_(True)
+    q = omodels.gtr_q(wl.exch, wl.freqs)
+    mats = omodels.p_t_eigh(q, wl.freqs, b.unsqueeze(-1) * wl.rates.reshape(1, 1, -1))
+    ref = opruning.treelik_discrete_rescaled(wl.tip_partials() + [None] * (wl.T - 1), wl.weights,
+                                             post, mats, wl.freqs.reshape(1, -1),
+                                             wl.props.reshape(-1, 1, 1)).reshape(-1)
+    ref.sum().backward()
+    job = TreeJob(post, list(wl.codes), wl.weights, 4, dtype=F64, device=gpu, tip_kind="states")
+    bd = bls.to(gpu).requires_grad_(True)
+    got = job.loglik(bd, wl.rates.reshape(1, -1), wl.props.reshape(1, -1),
+                     hm.normalize(hm.gtr_q(wl.exch, wl.freqs), wl.freqs), wl.freqs)
+    got.sum().backward()
+    g = bd.grad.cpu()
+    assert torch.isfinite(g).all()
+    assert math.isclose(float(got), float(ref), rel_tol=1e-12)
+    assert float((g - b.grad).norm() / b.grad.norm()) < 1e-9

@@ -229,3 +229,120 @@ def test_bench_refuses_more_than_one_gpu():
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_tip_cache_misses_on_new_tensors():
+    """A fresh tip tensor that reuses a freed tensor's id and address is a cache miss."""
+    import torch
+
+    from torchtree_amd import likelihood as hl
+
+    hits = 0
+    for i in range(40):
+        tips = [torch.full((4, 30), float(i % 7 + 1), dtype=torch.float64) for _ in range(3)]
+        tips[0][i % 4, 0] = 0.0
+        td = hl.get_tips(tips, "partials", 4, "cpu")
+        dense = td.table[td.tips.long()] if td.kind == 0 else td.tips
+        want = torch.stack(tips).permute(0, 2, 1)
+        assert torch.equal(dense, want)
+        hits += hl.get_tips(tips, "partials", 4, "cpu") is td
+        tips[1].mul_(2.0)  # an in-place write is a miss too
+        td2 = hl.get_tips(tips, "partials", 4, "cpu")
+        assert td2 is not td
+        del tips, td, td2
+    assert hits == 40
+
+
+def test_tip_codes_outside_table_rejected():
+    import torch
+
+    from torchtree_amd import likelihood as hl
+    from torchtree_amd._lib import HipTreeError
+
+    with pytest.raises(HipTreeError):
+        hl.TipData(0, torch.tensor([[0, 3]], dtype=torch.uint8), torch.ones(3, 4), 4, 2, 1)
+
+
+def test_gamma_site_model_batched_shape():
+    """A batched shape [K, 1] gives each sample its own gamma rates (not the first sample's)."""
+    import torch
+
+    from oracle import models as omodels
+    from torchtree_amd import treelikelihood_model as tm
+
+    dic = {}
+    site = tm.process_object({"type": "GammaSiteModel", "id": "sm", "categories": 4,
+                              "shape": {"type": "Parameter", "id": "a",
+                                        "tensor": [[0.5], [2.0], [7.0]]}}, dic)
+    assert tuple(site.sample_shape) == (3,)
+    rates, probs = site.rates_probs()
+    assert rates.shape == (3, 4)
+    for k, a in enumerate((0.5, 2.0, 7.0)):
+        want, p = omodels.gamma_rates_probs(a, 4)
+        assert torch.allclose(rates[k], want, rtol=1e-14)
+    assert torch.allclose(probs, p)
+
+
+def test_plugin_discovery_and_retarget():
+    """Replays torchtree's PluginManager.load_plugins (cli/plugin_manager.py:10-23) and the CLI's
+    load_arguments / process_tree_likelihood calls (cli/cli.py:59-60, cli/evolution.py:697-698)."""
+    import argparse
+    import importlib
+    import pkgutil
+
+    loaded = {}
+    for _finder, name, _ispkg in pkgutil.iter_modules([str(ROOT)]):
+        if name.startswith("torchtree_"):
+            m = importlib.import_module(name)
+            full = f"{name}.{m.__plugin__}".split(".")
+            loaded[m.__plugin__] = getattr(importlib.import_module(".".join(full[:-1])), full[-1])()
+    assert list(loaded) == ["plugin.HipTreePlugin"]
+    plugin = loaded["plugin.HipTreePlugin"]
+
+    parser = argparse.ArgumentParser()
+    sub = parser.add_subparsers()
+    for cmd in ("advi", "map", "mcmc", "hmc"):
+        sub.add_parser(cmd)
+    plugin.load_arguments(sub)
+    like = {"id": "like", "type": "TreeLikelihoodModel", "tree_model": "tree"}
+    plugin.process_tree_likelihood(parser.parse_args(["advi"]), like)
+    assert like["type"] == "TreeLikelihoodModel" and "precision" not in like
+    plugin.process_tree_likelihood(
+        parser.parse_args(["map", "--hiptree", "--hiptree_precision", "float32"]), like)
+    assert like["type"] == "torchtree_amd.treelikelihood_model.TreeLikelihoodModel"
+    assert like["precision"] == "float32"
+    # the type resolves the way torchtree's get_class does (core/utils.py:116-125)
+    parts = like["type"].split(".")
+    klass = getattr(importlib.import_module(".".join(parts[:-1])), parts[-1])
+    assert hasattr(klass, "from_json_safe") and hasattr(klass, "from_json")
+
+    # --engine torchtree_amd: cli/evolution.py imports <engine>.cli.evolution
+    evol = importlib.import_module("torchtree_amd.cli.evolution")
+    like2 = {"id": "like", "type": "TreeLikelihoodModel"}
+    evol.process_tree_likelihood(argparse.Namespace(hiptree_precision=None), like2)
+    assert like2["type"].endswith("treelikelihood_model.TreeLikelihoodModel")
+    with pytest.raises(ValueError):
+        evol.process_tree_likelihood(argparse.Namespace(), {"id": "x", "type": "JC69"})
+
+
+def test_site_pattern_indices_match_oracle():
+    """SitePattern ``indices`` (site_pattern.py:69-97, core/utils.py:404-419): the selected
+    columns, in the order given and with repeats, compressed like the whole alignment."""
+    import numpy as np
+
+    from oracle import data as odata
+    from torchtree_amd import treelikelihood_model as tm
+
+    idx = tm.parse_indices("3:40,7,0:60:3,100:,::17")
+    assert idx == [slice(3, 40), 7, slice(0, 60, 3), slice(100, None), slice(None, None, 17)]
+    rng = np.random.default_rng(8)
+    seqs = ["".join(rng.choice(list("ACGT-"), 130)) for _ in range(6)]
+    taxa = [tm.Taxon(f"t{i}", {}) for i in range(6)]
+    sp = tm.SitePattern(tm.Alignment([(f"t{i}", s) for i, s in enumerate(seqs)], taxa,
+                                     al.NucleotideType()), idx)
+    cols, w = sp.compressed()
+    ref, wref = odata.compress(seqs, indices=idx)
+    assert torch.equal(w, wref)
+    got = [["".join(chr(c) for c in cols[t, :, j]) for j in range(cols.shape[2])]
+           for t in range(6)]
+    assert [list(p) for p in ref] == got

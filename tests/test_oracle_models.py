@@ -106,6 +106,77 @@ def test_mg94_flags_counts():
     assert torch.all(syn * non == 0)
 
 
+# Hand-derived MG94 entries (codon.py:33-77) under the universal code. Sense-state indices are
+# 16*b1 + 4*b2 + b3 (A=0, C=1, G=2, T=3) minus the stops TAA (48), TAG (50), TGA (56) before it.
+# Each entry is the exchangeability times pi_j, with kappa on transitions (A<->G, C<->T), alpha
+# on synonymous and beta on non-synonymous one-step changes, and 1 for multi-step pairs.
+_A, _B, _K = 1.3, 0.3, 2.5
+MG94_ENTRIES = [
+    # (from, to, state i, state j, exchangeability)
+    ("AAA", "AAG", 0, 2, _K * _A),    # K -> K, A<->G: synonymous transition
+    ("AAA", "AAC", 0, 1, _B),         # K -> N, A<->C: non-synonymous transversion
+    ("AAA", "AGA", 0, 8, _K * _B),    # K -> R, A<->G: non-synonymous transition
+    ("AAA", "CCC", 0, 21, 1.0),       # three positions differ
+    ("AAA", "ACC", 0, 5, 1.0),        # two positions differ
+    ("CTA", "CTC", 28, 29, _A),       # L -> L, A<->C: synonymous transversion
+    ("TTA", "CTA", 57, 28, _K * _A),  # L -> L, T<->C at the first position
+    ("ATG", "ATA", 14, 12, _K * _B),  # M -> I, G<->A: non-synonymous transition
+    ("TAC", "TAT", 48, 49, _K * _A),  # Y -> Y across the TAA stop
+    ("TGG", "TGC", 55, 54, _B),       # W -> C, G<->C across three stops
+    ("TGG", "CGG", 55, 26, _K * _B),  # W -> R, T<->C
+    ("GTT", "TTT", 47, 60, _B),       # V -> F, G<->T: transversion
+]
+
+
+def _mg94_qs():
+    from torchtree_amd import models as hm
+
+    g = torch.Generator().manual_seed(5)
+    pi = torch.rand(61, generator=g, dtype=F64) + 0.2
+    pi = pi / pi.sum()
+    a, b, k = (torch.tensor([v], dtype=F64) for v in (_A, _B, _K))
+    return pi, models.mg94_q(a, b, k, pi), hm.mg94_q(a, b, k, pi)
+
+
+@pytest.mark.parametrize("which", ["oracle", "product"])
+def test_mg94_hand_derived_entries(which):
+    """Q_ij of MG94 with alpha, beta and kappa all different, against entries derived by hand
+    from the universal code; catches a wrong transition set, a swapped synonymous test or a
+    misplaced stop-codon shift (both the oracle and the engine's Q builder)."""
+    pi, qo, qp = _mg94_qs()
+    q = qo if which == "oracle" else qp
+    trip = models.codon_triplets()
+    sense = [c for i, c in enumerate(trip) if models.UNIVERSAL[i] != "*"]
+    for c1, c2, i, j, ex in MG94_ENTRIES:
+        assert sense[i] == c1 and sense[j] == c2
+        assert abs(float(q[i, j]) - ex * float(pi[j])) < 1e-15, (c1, c2)
+        assert abs(float(q[j, i]) - ex * float(pi[i])) < 1e-15, (c2, c1)
+    assert torch.allclose(q.sum(-1), torch.zeros(61, dtype=F64), atol=1e-14)
+
+
+def test_mg94_pair_totals():
+    """Unordered one-step sense pairs of the universal code, counted by mutating each codon
+    position: 263 in all, 67 of them synonymous. Transitions: each of the 64 codons has one
+    transition neighbour per position, 96 unordered pairs, less the 7 that touch a stop codon
+    (TAA-TAG, TAA-TGA, TAA-CAA, TAG-TGG, TAG-CAG, TGA-TGG, TGA-CGA): 89."""
+    ts, syn, non = models.mg94_masks()
+    assert int((syn + non).sum()) == 263
+    assert int(ts.sum()) == 89
+    assert int(syn.sum()) == 67
+    tab, trip = models.UNIVERSAL, models.codon_triplets()
+    n_ts = n_syn = 0
+    for a in range(64):
+        for p in range(3):
+            for base in "ACGT":
+                t = trip[a][:p] + base + trip[a][p + 1:]
+                b = trip.index(t)
+                if b <= a or tab[a] == "*" or tab[b] == "*":
+                    continue
+                n_ts += {trip[a][p], base} in ({"A", "G"}, {"C", "T"})
+                n_syn += tab[a] == tab[b]
+    assert (n_ts, n_syn) == (89, 67)
+
+
 @pytest.mark.parametrize("seed,S", [(0, 4), (1, 20), (2, 61)])
 def test_expm_and_identities(seed, S):
     from scipy.linalg import expm
@@ -205,3 +276,49 @@ def test_nonsymmetric_reduces_to_symmetric():
     assert torch.allclose(qn, qs, atol=1e-15)
     tau = torch.tensor([[0.05, 0.4], [1.1, 0.0]], dtype=F64)
     assert torch.allclose(models.p_t_expm(qn, pi, tau), models.p_t_eigh(qs, pi, tau), atol=1e-12)
+
+
+def _empirical_fixture():
+    import json
+    from pathlib import Path
+
+    d = json.loads((Path(__file__).parent / "golden" / "empirical_aa.json").read_text())
+    return {k: (torch.tensor(v["rates"], dtype=F64), torch.tensor(v["frequencies"], dtype=F64))
+            for k, v in d.items()}
+
+
+@pytest.mark.parametrize("name", ["LG", "WAG"])
+def test_empirical_tables_and_identities(name):
+    """LG / WAG (amino_acid.py:12-105): the engine's tables equal the reference's (fixture made
+    from its source text), and P(t) from them is a proper reversible transition matrix."""
+    from torchtree_amd import empirical
+
+    rates, pi = _empirical_fixture()[name]
+    r2, f2 = empirical.empirical(name)
+    assert torch.equal(rates, r2) and torch.equal(pi, f2)
+    assert rates.shape == (190,) and pi.shape == (20,) and bool((rates > 0).all())
+    assert abs(float(pi.sum()) - 1.0) < 2e-6
+    q = models.q_from_exchangeabilities(rates, pi)
+    qn = models.normalize_q(q, pi)
+    assert abs(float(-(torch.diagonal(qn) * pi).sum()) - 1.0) < 1e-14
+    t = torch.tensor([[0.0, 0.1, 0.35, 2.0]], dtype=F64)
+    P = models.p_t_eigh(q, pi, t)[0]  # [4, 20, 20]
+    assert torch.allclose(P[0], torch.eye(20, dtype=F64), atol=1e-13)
+    assert torch.allclose(P.sum(-1), torch.ones(4, 20, dtype=F64), atol=1e-13)
+    assert torch.allclose(pi @ P, pi.expand(4, 20), atol=1e-13)
+    assert torch.allclose(pi[:, None] * P, (pi[:, None] * P).transpose(-1, -2), atol=1e-14)
+    P45 = models.p_t_eigh(q, pi, torch.tensor([[0.45]], dtype=F64))[0, 0]
+    assert torch.allclose(P[1] @ P[2], P45, atol=1e-13)
+    from scipy.linalg import expm
+
+    assert np.allclose(P[3].numpy(), expm(qn.numpy() * 2.0), atol=1e-12)
+
+
+@pytest.mark.parametrize("inp,size", [([[1.0]], [1, 1, 20, 20]), ([[1.0, 1.0]], [1, 2, 20, 20]),
+                                      ([[1.0], [1.0]], [2, 1, 20, 20]),
+                                      ([[1.0, 1.0], [1.0, 1.0], [1.0, 1.0]], [3, 2, 20, 20])])
+def test_empirical_p_t_shapes(inp, size):
+    """test/test_substitution_model.py:283-295 (LG)."""
+    rates, pi = _empirical_fixture()["LG"]
+    P = models.p_t_eigh(models.q_from_exchangeabilities(rates, pi), pi, torch.tensor(inp, dtype=F64))
+    assert list(P.shape) == size

@@ -117,6 +117,9 @@ int hx_job_create(hx_plan_t plan, int S, int N, int C, int dtype, int slice_patt
   HX_REQUIRE(S >= 2 && S <= 64 && N >= 1 && C >= 1 && ncodes >= 1 && ncodes <= hx::kMaxCodes,
              HX_ERR_INVALID_ARGUMENT, "bad dimensions");
   HX_REQUIRE(dtype == HX_F32 || dtype == HX_F64, HX_ERR_INVALID_ARGUMENT, "bad dtype");
+  // The kernels index the code table with the raw code, so every code must name a row.
+  const uint8_t max_code = *std::max_element(codes, codes + (size_t)plan->T * N);
+  HX_REQUIRE(max_code < ncodes, HX_ERR_INVALID_ARGUMENT, "tip code outside the code table");
   auto* j = new hx_job_s();
   j->plan = plan;
   j->device = device;

@@ -73,11 +73,15 @@ __global__ void k_pmatrix_block(int S, int K, int B, int C, int eb, const double
   }
 }
 
-// (exp(a) - exp(b)) / (a - b) written as exp(b) * expm1(d) / d, d = a - b.
-__device__ __forceinline__ double exp_divdiff(double a, double b, double eb) {
-  const double d = a - b;
-  if (d == 0.0) return eb;
-  return eb * (expm1(d) / d);
+// (exp(a) - exp(b)) / (a - b) = exp(m) * (1 - exp(-d)) / d with m = max(a, b), d = |a - b|
+// (ea = exp(a), eb = exp(b)). Factoring out the larger exponent keeps every factor finite:
+// exp(m) is the larger of the two, and (1 - exp(-d)) / d lies in (0, 1]. The form
+// exp(b) * expm1(a - b) / (a - b) gives 0 * inf for b < -709 with a - b > 709.
+__device__ __forceinline__ double exp_divdiff(double a, double b, double ea, double eb) {
+  const double d = fabs(a - b);
+  const double em = a > b ? ea : eb;
+  if (d == 0.0) return em;
+  return em * (-expm1(-d) / d);
 }
 
 // Small S: one thread per (k, b, c). Y = L^T G R^T, dtau = sum_m lam_m e^{lam_m tau} Y_mm,
@@ -126,7 +130,7 @@ __global__ void k_pgrad_small(int K, int B, int C, int eb, const double* __restr
 #pragma unroll
       for (int i = 0; i < S; ++i) y = fma(le[i * S + m], tmp[i * S + n], y);
       if (m == n) dt = fma(la[m] * ex[m], y, dt);
-      xpart[idx * SS + m * S + n] = tau * y * exp_divdiff(a[m], a[n], ex[n]);
+      xpart[idx * SS + m * S + n] = tau * y * exp_divdiff(a[m], a[n], ex[m], ex[n]);
     }
   dtau[idx] = dt;
 }
@@ -172,7 +176,7 @@ __global__ void k_pgrad_block(int S, int K, int B, int C, int eb, const double* 
   double* X = xpart + idx * SS;
   for (int mn = threadIdx.x; mn < SS; mn += blockDim.x) {
     const int m = mn / S, n = mn % S;
-    X[mn] = tau * G[mn] * exp_divdiff(la[m] * tau, la[n] * tau, ex[n]);
+    X[mn] = tau * G[mn] * exp_divdiff(la[m] * tau, la[n] * tau, ex[m], ex[n]);
   }
   if (threadIdx.x == 0) {
     double t = 0.0;

@@ -29,6 +29,7 @@ from __future__ import annotations
 import collections
 import ctypes
 import math
+import weakref
 
 import numpy as np
 import torch
@@ -101,6 +102,9 @@ class TipData:
         self.S, self.N, self.T = S, N, T
         self.ncodes = 0 if table is None else table.shape[0]
         self._by_dtype = {}
+        if kind == HX_TIPS_CODES and tips.numel() and int(tips.max()) >= self.ncodes:
+            # the kernels index the table with the raw code
+            raise HipTreeError(f"tip code {int(tips.max())} outside a table of {self.ncodes} rows")
 
     def table_as(self, dtype):
         if self.table is None:
@@ -142,24 +146,37 @@ class TipData:
         return cls(HX_TIPS_CODES, codes.to(torch.uint8).contiguous(), table.to(device), S, N, T)
 
 
-_TIP_CACHE: "collections.OrderedDict[tuple, TipData]" = collections.OrderedDict()
+_TIP_CACHE: "collections.OrderedDict[tuple, tuple]" = collections.OrderedDict()
 
 
 def _tip_key(tips, kind, device):
-    return (kind, device) + tuple((id(t), t.data_ptr(), t._version, tuple(t.shape)) for t in tips)
+    return (kind, device) + tuple((id(t), t.data_ptr(), tuple(t.shape)) for t in tips)
+
+
+def _tips_unchanged(refs, versions, tips):
+    """The cached entry was built from these very tensor objects, unmodified since.
+
+    An id or a storage address alone can be reused by a new tensor once the old
+    one is freed; a weak reference cannot: it dies with its tensor. In-place
+    writes (through the tensor or any view of it) bump ``_version``.
+    """
+    return all(r() is t and t._version == v for r, v, t in zip(refs, versions, tips))
 
 
 def get_tips(tips, kind: str, S: int, device: int) -> TipData:
     key = _tip_key(tips, kind, device)
-    td = _TIP_CACHE.get(key)
-    if td is None:
-        if kind == "states":
-            td = TipData.from_states(tips, S, device)
-        else:
-            td = TipData.from_partials(tips, device)
-        _TIP_CACHE[key] = td
-        while len(_TIP_CACHE) > 16:
-            _TIP_CACHE.popitem(last=False)
+    hit = _TIP_CACHE.get(key)
+    if hit is not None and _tips_unchanged(hit[0], hit[1], tips):
+        _TIP_CACHE.move_to_end(key)
+        return hit[2]
+    if kind == "states":
+        td = TipData.from_states(tips, S, device)
+    else:
+        td = TipData.from_partials(tips, device)
+    _TIP_CACHE[key] = ([weakref.ref(t) for t in tips], [t._version for t in tips], td)
+    _TIP_CACHE.move_to_end(key)
+    while len(_TIP_CACHE) > 16:
+        _TIP_CACHE.popitem(last=False)
     return td
 
 

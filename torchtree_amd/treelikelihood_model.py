@@ -17,7 +17,8 @@ every parameter tensor (``requires_grad``):
 * ``UnRootedTreeModel`` (``tree_model.py:225-344``), ``TimeTreeModel`` (349-463),
   ``ReparameterizedTimeTreeModel`` with ``ratios``/``root_height`` (527-707).
 * ``JC69``, ``HKY``, ``GTR`` (``nucleotide.py``), ``GeneralJC69``,
-  ``GeneralSymmetricSubstitutionModel``, ``EmpiricalSubstitutionModel``,
+  ``GeneralSymmetricSubstitutionModel``, ``EmpiricalSubstitutionModel``, ``LG``, ``WAG``
+  (``amino_acid.py:12-105``),
   ``GeneralNonSymmetricSubstitutionModel`` (``general.py``; P = expm(Q t) through
   torch, as ``abstract.py:88-94``), ``MG94`` (``codon.py``).
 * ``ConstantSiteModel``, ``InvariantSiteModel``, ``WeibullSiteModel``
@@ -37,6 +38,7 @@ import torch
 
 from . import alignment as al
 from . import models as hm
+from .empirical import empirical
 from .engine import TreeJob
 from .likelihood import TipData, _PruneFn
 from .tree import (Topology, heights_to_ratios, leaf_heights_from_dates, ratio_bounds,
@@ -192,13 +194,38 @@ def _alignment(data, dic):
     return Alignment(seqs, taxa, dt)
 
 
+def parse_indices(text):
+    """``"0:100,250,300::2"`` -> [slice(0, 100), 250, slice(300, None, 2)], as the reference's
+    ``string_to_list_index`` reads each comma-separated item (core/utils.py:404-419)."""
+    out = []
+    for item in text.split(","):
+        parts = item.split(":")
+        if len(parts) == 1:
+            out.append(int(parts[0]))
+            continue
+        start = int(parts[0]) if parts[0] != "" else None
+        stop = int(parts[1]) if parts[1] != "" else None
+        step = int(parts[2]) if len(parts) == 3 and parts[2] != "" else None
+        out.append(slice(start, stop, step))
+    return out
+
+
 class SitePattern:
-    def __init__(self, alignment):
+    """Alignment columns, optionally a selection of them (``indices``, site_pattern.py:69-97:
+    each index or slice of every sequence, concatenated in the order given)."""
+
+    def __init__(self, alignment, indices=None):
         self.alignment = alignment
+        self.indices = indices
+
+    def sequences(self):
+        seqs = [s for _, s in self.alignment.sequences]
+        if self.indices is None:
+            return seqs
+        return ["".join(s[i] for i in self.indices) for s in seqs]
 
     def compressed(self):
-        seqs = [s for _, s in self.alignment.sequences]
-        return al.compress_bytes(seqs, self.alignment.data_type.size)
+        return al.compress_bytes(self.sequences(), self.alignment.data_type.size)
 
 
 class AttributePattern:
@@ -235,9 +262,15 @@ def _attribute_pattern(data, dic):
 
 @register("SitePattern")
 def _site_pattern(data, dic):
-    if data.get("indices"):
-        raise JSONParseError("SitePattern `indices' is not supported by this engine")
-    return SitePattern(process_object(data["alignment"], dic))
+    alignment = process_object(data["alignment"], dic)
+    indices = data.get("indices")
+    if indices is None:
+        return SitePattern(alignment)
+    if alignment.data_type.size > 1:
+        # the reference selects characters of each sequence, which it cannot do for codons
+        # (compress zips their triplets first, site_pattern.py:82-91)
+        raise JSONParseError("SitePattern `indices' applies to one-character data types only")
+    return SitePattern(alignment, parse_indices(indices))
 
 
 # ------------------------------------------------------------ tree models
@@ -442,6 +475,21 @@ def _empirical(data, dic):
                       lambda: ())
 
 
+def _fixed_empirical(name):
+    """LG / WAG (amino_acid.py:12-105): an EmpiricalSubstitutionModel with a fixed table."""
+
+    def build(data, dic):
+        rates, freqs = empirical(name)
+        q = hm.q_from_exchangeabilities(rates, freqs)
+        return SubstModel(20, lambda: q, lambda: freqs, lambda: ())
+
+    return build
+
+
+register("LG")(_fixed_empirical("LG"))
+register("WAG")(_fixed_empirical("WAG"))
+
+
 @register("MG94")
 def _mg94(data, dic):
     dt = _data_type(data["data_type"], dic)
@@ -511,12 +559,19 @@ def _gamma(data, dic):
     cats = int(data["categories"])
 
     def fn():
-        if shape.tensor.requires_grad:
+        """Rates [*batch, C] (one row per sample of a batched shape [*batch, 1]), probs [C]."""
+        x = shape.tensor
+        if x.requires_grad:
             raise JSONParseError("GammaSiteModel shape gradients are not provided")
-        r, p = hm.gamma_rates(float(shape.tensor.reshape(-1)[0]), cats)
-        return r, p
+        batch = tuple(x.shape[:-1])
+        flat = x.detach().reshape(-1)
+        if x.shape[-1:] != (1,) and x.dim() > 0:
+            raise JSONParseError("GammaSiteModel shape must have a trailing dimension of 1")
+        rows = [hm.gamma_rates(float(v), cats) for v in flat.tolist()]
+        rates = torch.stack([r for r, _ in rows]).reshape(batch + (cats,))
+        return rates, rows[0][1]
 
-    return SiteModel(fn, lambda: ())
+    return SiteModel(fn, lambda: _shape_of(shape))
 
 
 # ---------------------------------------------------------- branch models
@@ -550,12 +605,97 @@ def _simple(data, dic):
 # ------------------------------------------------------- the likelihood
 
 
+# ------------------------------------------- objects built by torchtree itself
+#
+# A JSON object may name sub-objects that torchtree's own loader built earlier (a string id
+# resolved through ``dic``). They are used through the methods torchtree's TreeLikelihoodModel
+# calls on them (tree_likelihood.py:305-356): ``branch_lengths()`` / ``postorder`` / ``taxa``
+# (TreeModel), ``q()`` / ``frequencies`` (SubstitutionModel, abstract.py:15-97), ``rates()`` /
+# ``probabilities()`` (SiteModel, site_model.py:13-35), ``rates`` (BranchModel), and
+# ``compute_tips_partials`` / ``compute_tips_states`` (SitePattern, site_pattern.py:27-31).
+
+
+def _shape_attr(obj):
+    return tuple(getattr(obj, "sample_shape", ()) or ())
+
+
+class _ForeignTree:
+    def __init__(self, obj):
+        self.obj = obj
+        post = np.asarray([list(map(int, r)) for r in obj.postorder], dtype=np.int32).reshape(-1, 3)
+        self.topo = type("Topo", (), {"T": post.shape[0] + 1, "B": 2 * post.shape[0],
+                                      "postorder": post})()
+
+    def branch_lengths(self):
+        return self.obj.branch_lengths()
+
+    @property
+    def sample_shape(self):
+        return _shape_attr(self.obj)
+
+
+class _ForeignSubst:
+    def __init__(self, obj):
+        self.obj = obj
+        self.state_count = obj.frequencies.shape[-1]
+        self.nonsymmetric = any("NonSymmetric" in k.__name__ for k in type(obj).__mro__)
+
+    def frequencies(self):
+        return self.obj.frequencies
+
+    def q_norm(self):
+        return hm.normalize(self.obj.q(), self.obj.frequencies)
+
+    @property
+    def sample_shape(self):
+        return _shape_attr(self.obj)
+
+
+class _ForeignSite:
+    def __init__(self, obj):
+        self.obj = obj
+
+    def rates_probs(self):
+        return self.obj.rates(), self.obj.probabilities()
+
+    @property
+    def sample_shape(self):
+        return _shape_attr(self.obj)
+
+
+class _ForeignClock:
+    def __init__(self, obj):
+        self.obj = obj
+
+    def rates(self):
+        r = self.obj.rates
+        return r() if callable(r) else r
+
+    @property
+    def sample_shape(self):
+        return _shape_attr(self.obj)
+
+
+def _adapt(obj, ours, foreign):
+    return obj if obj is None or isinstance(obj, ours) else foreign(obj)
+
+
 class TreeLikelihoodModel:
-    """HIP-backed ``TreeLikelihoodModel``: ``model()`` -> log-likelihood [*sample_shape, 1]."""
+    """HIP-backed ``TreeLikelihoodModel``: ``model()`` -> log-likelihood [*sample_shape, 1].
+
+    Sub-models are this module's objects (built from inline JSON) or torchtree's own objects
+    (resolved by id from ``dic``), used through the methods torchtree's model calls on them.
+    The result lives on the device of the branch lengths, as in the reference; nothing is
+    copied to the host unless the parameters are host tensors.
+    """
 
     def __init__(self, id_, site_pattern, tree_model, subst_model, site_model, clock_model=None,
                  use_ambiguities=False, use_tip_states=False, precision=None, device=None):
         self.id = id_
+        tree_model = _adapt(tree_model, (UnRootedTreeModel, TimeTreeModel), _ForeignTree)
+        subst_model = _adapt(subst_model, SubstModel, _ForeignSubst)
+        site_model = _adapt(site_model, SiteModel, _ForeignSite)
+        clock_model = _adapt(clock_model, ClockModel, _ForeignClock)
         self.site_pattern, self.tree_model = site_pattern, tree_model
         self.subst_model, self.site_model, self.clock_model = subst_model, site_model, clock_model
         if device is None:
@@ -564,17 +704,30 @@ class TreeLikelihoodModel:
         if precision is None:
             precision = torch.get_default_dtype()
         self.dtype = _DTYPES[precision] if isinstance(precision, str) else precision
+        topo = tree_model.topo
         if isinstance(site_pattern, AttributePattern):
             dt = site_pattern.data_type
             codes, table, weights = site_pattern.tip_codes(use_ambiguities, use_tip_states)
-        else:
+        elif isinstance(site_pattern, SitePattern):
             cols, weights = site_pattern.compressed()
             dt = site_pattern.alignment.data_type
             codes, table = al.encode_tips(cols, dt, use_ambiguities, use_tip_states)
+        else:  # torchtree's SitePattern: its own tip tensors, in taxa order
+            S = subst_model.state_count
+            if use_tip_states:
+                states, weights = site_pattern.compute_tips_states()
+                tips = TipData.from_states(list(states)[:topo.T], S, self.device)
+            else:
+                partials, weights = site_pattern.compute_tips_partials(use_ambiguities)
+                tips = TipData.from_partials(list(partials)[:topo.T], self.device)
+            self._finish(topo, tips, torch.as_tensor(weights), S)
+            return
         S = dt.state_count
-        topo = tree_model.topo
         tips = TipData(0, codes.contiguous().to(self.device), table.to(self.device), S,
                        codes.shape[1], topo.T)
+        self._finish(topo, tips, weights, S)
+
+    def _finish(self, topo, tips, weights, S):
         self.weights = weights
         self.job = TreeJob([tuple(int(x) for x in r) for r in topo.postorder], tips, weights, S,
                            dtype=self.dtype, device=self.device)
@@ -595,7 +748,7 @@ class TreeLikelihoodModel:
         bl = self.tree_model.branch_lengths()
         if self.clock_model is None:
             bl = bl.expand((K,) + bl.shape[-1:])
-            bls = torch.cat((bl, torch.zeros(K, 1, dtype=bl.dtype)), -1)
+            bls = torch.cat((bl, torch.zeros(K, 1, dtype=bl.dtype, device=bl.device)), -1)
         else:
             bls = (self.clock_model.rates() * bl).expand((K,) + bl.shape[-1:])
         rates, probs = self.site_model.rates_probs()
@@ -603,14 +756,15 @@ class TreeLikelihoodModel:
         probs = probs.reshape(-1, probs.shape[-1]).expand(K, -1) if probs.dim() > 1 else probs.unsqueeze(0)
         q = self.subst_model.q_norm()
         f = self.subst_model.frequencies()
+        out_dev = bl.device
         if self.subst_model.nonsymmetric:
             ll = self._loglik_expm(bls, rates, probs, q, f)
-            return ll.to(torch.get_default_dtype()).cpu().reshape(shape + (1,))
+            return ll.to(out_dev, torch.get_default_dtype()).reshape(shape + (1,))
         if q.dim() == 3 and q.shape[0] == 1:
             q, f = q[0], f.reshape(-1, f.shape[-1])[0]
         ll = self.job.loglik(bls.to(torch.float64), rates.to(torch.float64),
                              probs.to(torch.float64), q.to(torch.float64), f.to(torch.float64))
-        out = ll.to(torch.get_default_dtype()).cpu()
+        out = ll.to(out_dev, torch.get_default_dtype())
         return out.reshape(shape + (1,))
 
     def _loglik_expm(self, bls, rates, probs, q, f):
@@ -626,6 +780,16 @@ class TreeLikelihoodModel:
         f2 = f.to(dev, self.dtype).reshape(-1, f.shape[-1]).contiguous()
         p2 = probs.to(dev, self.dtype).contiguous()
         return _PruneFn.apply(mats, f2, p2, self.job.pruning)
+
+    @classmethod
+    def from_json_safe(cls, data, dic):
+        """torchtree's loader entry (core/serializable.py:33-61): a missing key is a
+        JSONParseError naming the key and the object."""
+        try:
+            return cls.from_json(data, dic)
+        except KeyError as e:
+            raise JSONParseError(f"Missing key `{e.args[0]}' for object of type "
+                                 f"`TreeLikelihoodModel' with ID `{data.get('id')}'") from None
 
     @classmethod
     def from_json(cls, data, dic):
