@@ -23,6 +23,7 @@ import json
 import math
 import os
 import statistics
+import subprocess
 import sys
 import time
 from pathlib import Path
@@ -48,15 +49,21 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-patterns", type=int, default=10000)
     ap.add_argument("--cpu-samples", type=int, default=2)
-    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--config", default="c2", choices=("c1", "c2", "c3", "c4a", "c4b"),
                     help="workload of the timed step (c2 is the headline configuration)")
+    ap.add_argument("--products", default="fp32", choices=("fp32", "bf16x3"),
+                    help="dense-state products of the FP32 step: FP32 matrix cores, or split "
+                         "BF16 (hi + lo halves, three products, FP32 accumulation)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-mix measurement")
     ap.add_argument("--serve-jobs", type=int, default=200)
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--serve-plain-limit", type=float, default=150.0,
                     help="seconds of sequential plain FP64 evaluation (stats over completed requests)")
-    ap.add_argument("--serve-streams", type=int, default=4)
+    ap.add_argument("--serve-streams", type=int, default=8)
+    ap.add_argument("--serve-graph-bytes", type=int, default=64 << 20,
+                    help="requests with at most this much workspace replay a captured HIP graph "
+                         "(0: every request launches eagerly)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run one device-to-device copy of BYTES (a known-size kernel "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -185,7 +192,8 @@ def main():
         return total, lat
 
     # ---- FP32 served-precision path (the measured value)
-    job32 = TreeJob(post, tips, wl.weights, S, dtype=torch.float32, device=dev)
+    job32 = TreeJob(post, tips, wl.weights, S, dtype=torch.float32, device=dev,
+                    products=args.products)
     step32 = make_step(job32)
     total, lat = timed(step32, args.steps, args.warmup)
     ms = total / args.steps * 1e3
@@ -227,6 +235,7 @@ def main():
             "workload": DESCRIPTIONS[args.config].format(T=wl.T, N=wl.N, K=wl.K),
             "taxa": wl.T, "patterns": wl.N, "categories": wl.C, "samples": wl.K, "states": S,
             "partials": "fp32 + power-of-two rescaling", "seed": args.seed,
+            "dense_products": args.products if S != 4 else "fp32",
         },
         "p50_ms": statistics.median(lat),
         "p99_ms": _pct(lat, 99),
@@ -308,8 +317,8 @@ def serving_bench(args, dev):
     specs = synthetic.serving_mix(args.serve_jobs, seed=5)
     log(f"serving mix: {len(specs)} jobs generated in {time.time()-t0:.1f}s")
     budget = 200 << 30
-    ws = 40 << 30
-    srv = serving.Server(budget, n_streams=args.serve_streams, ws_bytes_per_stream=ws,
+    srv = serving.Server(budget, n_streams=args.serve_streams, pool_bytes=160 << 30,
+                         max_request_bytes=40 << 30, graph_max_bytes=args.serve_graph_bytes,
                          device=dev.index or 0)
     jobs32, jobs64 = [], []
     for sp in specs:
@@ -348,10 +357,12 @@ def serving_bench(args, dev):
     log(f"served: {st}")
     plain = requests(jobs64, stream)
     serving.run_sequential(plain[: min(n, 5)], device=dev.index or 0)  # warm-up
+    # the plain path under the same arrivals, one request at a time, first come first served:
+    # latencies include its queueing exactly as the server's do
     plain = requests(jobs64, stream)
     pl = serving.run_sequential(plain, device=dev.index or 0, time_limit=args.serve_plain_limit,
-                                progress=prog)
-    log(f"plain fp64 sequential: {pl}")
+                                progress=prog, realtime=True)
+    log(f"plain fp64 sequential, same arrivals: {pl}")
     errs, gerrs = [], []
     for a, b in zip(served, plain):
         if b.loglik is None or math.isnan(b.t_done):
@@ -363,10 +374,14 @@ def serving_bench(args, dev):
         "workload": f"{len(specs)} resident jobs (T~U{{16..1024}}, N log-uniform 100-100000, "
                     f"DNA/AA/codon {kinds.count('dna')}/{kinds.count('aa')}/{kinds.count('codon')} "
                     f"requests), K in {{1,8,32}}, Poisson arrivals at {rate:.1f}/s (80% of capacity), "
-                    f"{args.serve_streams} streams",
+                    f"{args.serve_streams} streams, {srv.pool.size >> 30} GiB workspace pool",
         "requests": st["requests"],
         "served_updates_per_s": st["updates_per_s"],
         "served_p50_ms": st["p50_ms"], "served_p99_ms": st["p99_ms"],
+        "served_counters": {k: st.get(k, 0) for k in ("admitted", "bypassed", "rejected",
+                                                     "cancelled", "queued_peak", "graphed",
+                                                     "pool_peak_bytes")},
+        "plain_same_arrivals": True,
         "capacity_updates_per_s": cap["updates_per_s"],
         "plain_fp64_sequential_updates_per_s": pl["updates_per_s"],
         "plain_fp64_requests_timed": pl["requests"],
@@ -419,7 +434,11 @@ def cpu_baseline(args, dev):
     from torchtree_amd.engine import TreeJob
     from torchtree_amd.likelihood import TipData
 
-    cores = min(16, len(os.sched_getaffinity(0)))
+    # every core this process may run on, within the CPU share the host grants it (the
+    # OMP_NUM_THREADS it exports, 16 per GPU on the MI355X hosts)
+    affinity = len(os.sched_getaffinity(0))
+    share = int(os.environ.get("OMP_NUM_THREADS", affinity) or affinity)
+    cores = max(1, min(affinity, share))
     torch.set_num_threads(cores)
     wl = synthetic.config2(seed=args.seed, N=args.cpu_patterns, K=args.cpu_samples, T=args.taxa)
     post = [tuple(int(x) for x in r) for r in wl.topo.postorder]
@@ -453,10 +472,16 @@ def cpu_baseline(args, dev):
     got.sum().backward()
     rel = float(((got.detach().cpu() - ref_ll).abs() / ref_ll.abs()).max())
     grel = float((b.grad.cpu() - ref_g).norm() / ref_g.norm())
-    try:
-        model = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
+    model = "unknown"
+    try:  # lscpu's model name, else /proc/cpuinfo's
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        names = [l.split(":", 1)[1].strip() for l in out.splitlines() if l.startswith("Model name")]
+        if not names:
+            names = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo")
+                     if l.startswith("model name")]
+        model = names[0] if names else model
     except Exception:
-        model = "unknown"
+        pass
     return {
         "value": upd / med, "unit": "site-partial updates/s", "cores": cores, "kind": "port",
         "sample": f"FP64 torch oracle loglik+autograd grad, T={wl.T}, N={wl.N} patterns, "

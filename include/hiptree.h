@@ -58,7 +58,14 @@ typedef enum {
   HX_ERR_NOT_FOUND = 8
 } hx_status;
 
-enum { HX_F32 = 0, HX_F64 = 1 };
+/* Precision of a call (the `dtype` argument). HX_F32 and HX_F64: partials,
+ * P and every product in that type. HX_F32_BF16X3: partials and P in FP32 (R =
+ * float in every array convention), and the dense-state contractions (S != 4,
+ * at most 4 categories) on the BF16 matrix cores in split precision: each
+ * operand x carried as hi = bf16(x) plus lo = bf16(x - hi), each product formed
+ * as lo*hi + hi*lo + hi*hi with FP32 accumulation. Four-state calls run as
+ * HX_F32. */
+enum { HX_F32 = 0, HX_F64 = 1, HX_F32_BF16X3 = 2 };
 enum { HX_TIPS_CODES = 0, HX_TIPS_DENSE = 1 };
 
 /* Human-readable text for a status, plus the last error detail. */
@@ -162,6 +169,31 @@ int hx_pmatrix_grad(int S, int K, int B, int C, int eigen_batched,
                     const double* dmats, double* out_dbls, double* out_drates,
                     double* out_dq, void* scratch, hx_stream_t stream);
 
+/* ------------------------------------------ P matrices of non-reversible models */
+/* mats[k][b][c] = exp(Q_e tau), tau = bls[k][b] * rates[k or 0][c], e = k if
+ * q_batched else 0, for any rate matrix Q [E][S][S] (double, normalised; S <= 32),
+ * in `dtype`: scaling and squaring of a degree-18 Taylor polynomial in FP64.
+ * Replaces NonSymmetricSubstitutionModel.p_t (substitution_model/abstract.py:88-94),
+ * which calls torch.matrix_exp. */
+int hx_pmatrix_expm(int S, int K, int B, int C, int q_batched, const double* Q,
+                    const double* bls, const double* rates, int rates_batched, void* mats,
+                    int dtype, hx_stream_t stream);
+/* dP/dtau = Q_e exp(Q_e tau), same arguments and layout as hx_pmatrix_expm. */
+int hx_pmatrix_expm_dtau(int S, int K, int B, int C, int q_batched, const double* Q,
+                         const double* bls, const double* rates, int rates_batched,
+                         void* mats_dtau, int dtype, hx_stream_t stream);
+/* Chain rule for P = exp(Q tau) from dmats = d loglik / d mats [K][B][C][S][S]
+ * (double): out_dbls [K][B], out_drates [K][C], out_dq [E][S][S] = sum of
+ * tau L(tau Q^T, G) with L the Frechet derivative of exp, from the upper right
+ * block of exp([[tau Q^T, G], [0, tau Q^T]]). The autograd backward of
+ * torch.matrix_exp in NonSymmetricSubstitutionModel.p_t (abstract.py:92). `scratch`
+ * needs hx_pmatrix_expm_grad_scratch_bytes(S, K, B, C) bytes of device memory. */
+size_t hx_pmatrix_expm_grad_scratch_bytes(int S, int K, int B, int C);
+int hx_pmatrix_expm_grad(int S, int K, int B, int C, int q_batched, const double* Q,
+                         const double* bls, const double* rates, int rates_batched,
+                         const double* dmats, double* out_dbls, double* out_drates, double* out_dq,
+                         void* scratch, hx_stream_t stream);
+
 /* ------------------------------------------------------------ resident jobs */
 /* A job keeps one alignment resident on a device for repeated requests:
  * uint8 tip codes [T][N] and their table [ncodes][S] (host pointers, copied
@@ -196,6 +228,29 @@ size_t hx_job_workspace_bytes_for(int taxa, int S, int slice_patterns, int C, in
  * with respect to branch lengths. */
 int hx_job_eval(hx_job_t job, int K, const double* bls, double* out_loglik, double* out_dbls,
                 void* workspace, size_t workspace_bytes, hx_stream_t stream);
+
+/* One request that brings its own substitution and site model: P from the
+ * given FP64 eigensystem (lambda [E][S], left [E][S][S], right [E][S][S] as in
+ * hx_pmatrix, E = K if eigen_batched else 1), category rates [R][C],
+ * proportions [R][C] and root frequencies [R][S] (R = K if model_batched else
+ * 1), all device doubles, in place of the ones the job was registered with.
+ * This is TreeLikelihoodModel._call (tree_likelihood.py:313-356) re-evaluating
+ * p_t from the current parameters on every call (line 346), as an ADVI or HMC
+ * job over a GTR model does, with its backward. Outputs (device doubles; all
+ * but out_loglik may be NULL):
+ *   out_loglik [K]; out_dbls [K][B]; out_drates [K][C];
+ *   out_dq [E][S][S]   d loglik / d normalised Q (through P only);
+ *   out_dfreqs [K][S]  d loglik / d root frequencies; out_dprops [K][C].
+ * The gradients with respect to a model's own parameters (exchangeabilities,
+ * frequencies through Q, a site-model shape) follow from these by the chain
+ * rule on the host. Workspace: hx_job_model_workspace_bytes(job, K, eigen_batched). */
+size_t hx_job_model_workspace_bytes(hx_job_t job, int K, int eigen_batched);
+int hx_job_eval_model(hx_job_t job, int K, const double* bls, int eigen_batched,
+                      const double* lambda, const double* left, const double* right,
+                      const double* rates, const double* props, const double* freqs,
+                      int model_batched, double* out_loglik, double* out_dbls, double* out_drates,
+                      double* out_dq, double* out_dfreqs, double* out_dprops, void* workspace,
+                      size_t workspace_bytes, hx_stream_t stream);
 
 /* Checks on the device the operand/accumulator lane layout the MFMA
  * pruning kernels assume for v_mfma_f32_16x16x4f32; *mismatches = number of

@@ -18,8 +18,9 @@
   is bitwise the same alone and in the concurrent mix.
 
 Tolerances (FP32 with rescaling against FP64): log-likelihood relative 1e-6, gradients relative
-1e-5 norm-wise per parameter group; FP64 against FP64 1e-12 and 1e-10; central differences
-relative 1e-5 norm-wise.
+1e-5 norm-wise per parameter group; FP64 against FP64 1e-12 and 1e-10 (1e-9 through the
+model coordinates); central differences relative 1e-5 norm-wise, or three times their own
+rounding-noise floor where that is larger (a 1e-6 step on a log-likelihood of ~10^6).
 """
 import math
 import time
@@ -85,7 +86,7 @@ def test_config1_full_with_gradient(gpu):
         bd = wl.bls.to(gpu).requires_grad_(True)
         got = job.loglik(bd, wl.rates.reshape(1, -1), wl.props.reshape(1, -1), wl.q_norm, wl.freqs)
         got.sum().backward()
-        assert math.isclose(float(got[0]), float(ref[0]), rel_tol=rl), (dtype, float(got[0]))
+        assert math.isclose(float(got[0].detach()), float(ref[0].detach()), rel_tol=rl), dtype
         assert torch.isfinite(bd.grad).all()
         assert _rel(bd.grad, b.grad) < rg, (dtype, _rel(bd.grad, b.grad))
 
@@ -201,11 +202,21 @@ def test_config3_1024_taxa_k16_and_central_differences(gpu):
         up[:, j] += h
         dn[:, j] -= h
         fd_f[:, j] = (f(xr, up, wl.bls) - f(xr, dn, wl.bls)) / (2 * h)
+    # A central difference carries the rounding of two FP64 evaluations of logL divided by 2h:
+    # about 200 ulp of |logL| (the pattern sums) over 2e-6. Its relative accuracy on a group of
+    # coordinates is that noise over the group's gradient norm, written out here; the
+    # tolerance is the larger of 1e-5 and three times it.
+    f0 = f(xr, xf, wl.bls)
+    noise = 200 * 2.2e-16 * float(f0.abs().max()) / (2 * h)
     for dtype in (F64, torch.float32):
         _, gb, gxr, gxf = grads[dtype]
-        eb, er, ef = _rel(gb[:, branches], fd_b), _rel(gxr, fd_r), _rel(gxf, fd_f)
-        _flush(f"  {dtype} vs central differences: bls {eb:.2e} rates {er:.2e} freqs {ef:.2e}")
-        assert max(eb, er, ef) < 1e-5, (dtype, eb, er, ef)
+        for name, g, fd in (("bls", gb[:, branches], fd_b), ("rates", gxr, fd_r),
+                            ("freqs", gxf, fd_f)):
+            err = _rel(g, fd)
+            tol = max(1e-5, 3 * noise * math.sqrt(fd.numel()) / float(fd.norm()))
+            _flush(f"  {dtype} vs central differences, {name}: {err:.2e} (tolerance {tol:.1e}, "
+                   f"|logL| {float(f0.abs().max()):.3e})")
+            assert err < tol, (dtype, name, err, tol)
 
 
 def test_config4a_amino_acid_full_size(gpu):
@@ -229,7 +240,7 @@ def test_config4a_amino_acid_full_size(gpu):
     ref, rgb = _oracle_batched(wl, [0], lambda e, f: (omodels.q_from_exchangeabilities(e, f), f),
                                (ex_o, fr_o))
     _flush(f"  oracle {time.time() - t0:.1f}s")
-    assert math.isclose(float(ll[0]), float(ref[0]), rel_tol=1e-6)
+    assert math.isclose(float(ll[0].detach()), float(ref[0]), rel_tol=1e-6)
     assert _rel(bd.grad, rgb) < 1e-5
     assert _rel(ex.grad, ex_o.grad) < 1e-5
     assert _rel(fr.grad, fr_o.grad) < 1e-5
@@ -257,7 +268,7 @@ def test_config4b_codon_full_size(gpu):
     ref, rgb = _oracle_batched(
         wl, [0], lambda a, f: (omodels.mg94_q(a[0:1], a[1:2], a[2:3], f), f), (p_o, fr_o))
     _flush(f"  oracle {time.time() - t0:.1f}s")
-    assert math.isclose(float(ll[0]), float(ref[0]), rel_tol=1e-6)
+    assert math.isclose(float(ll[0].detach()), float(ref[0]), rel_tol=1e-6)
     assert _rel(bd.grad, rgb) < 1e-5
     assert _rel(p.grad, p_o.grad) < 1e-5
     assert _rel(fr.grad, fr_o.grad) < 1e-5
@@ -274,7 +285,7 @@ def test_config5_mix_alone_equals_mix(gpu):
     specs += [synthetic.random_job_spec("dna", 1024, 100_000, rng, name="dna-max"),
               synthetic.random_job_spec("aa", 1024, 100_000, rng, name="aa-max"),
               synthetic.random_job_spec("codon", 1024, 100_000, rng, name="codon-max")]
-    srv = serving.Server(200 << 30, n_streams=4, ws_bytes_per_stream=40 << 30,
+    srv = serving.Server(200 << 30, n_streams=8, pool_bytes=160 << 30,
                          device=gpu.index or 0)
     jobs = []
     for sp in specs:

@@ -353,7 +353,8 @@ class _TorchtreeLikeSitePattern:
 
 
 @pytest.mark.parametrize("use_tip_states", [False, True])
-def test_json_with_objects_built_by_torchtree(gpu, tiny_newick, tiny_fasta, use_tip_states):
+def test_json_with_objects_built_by_torchtree(gpu, default_f64, tiny_newick, tiny_fasta,
+                                              use_tip_states):
     """Sub-objects resolved by id from ``dic`` that torchtree's own loader would have built
     (stand-ins with the same methods): the tiny golden -83.329016, via from_json_safe."""
     print(f"case: JSON with torchtree-built objects, tip states {use_tip_states}", flush=True)
@@ -384,8 +385,17 @@ def _sequences(S, T, L, rng):
     return ["".join(rng.choice(pool, L)) for _ in range(T)]
 
 
+@pytest.fixture
+def default_f64():
+    """torchtree's runner default dtype (torchtree.py:43-48): the model returns it."""
+    prev = torch.get_default_dtype()
+    torch.set_default_dtype(torch.float64)
+    yield
+    torch.set_default_dtype(prev)
+
+
 @pytest.mark.parametrize("model", ["LG", "WAG", "HKY", "EmpiricalSubstitutionModel"])
-def test_json_substitution_models(gpu, model):
+def test_json_substitution_models(gpu, default_f64, model):
     """LG, WAG (amino_acid.py:12-105), HKY (nucleotide.py:238-264) and a 4-state
     EmpiricalSubstitutionModel (general.py:294-361) through JSON, with a Weibull site model:
     log-likelihood and gradients (branch lengths, Weibull shape, and HKY's kappa and

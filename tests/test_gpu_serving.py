@@ -41,7 +41,7 @@ def test_alone_equals_mix_and_slicing(gpu):
                 for i, (j, K, s, t) in enumerate(stream)]
     alone = requests()
     serving.run_sequential(alone, device=gpu.index or 0)
-    srv = serving.Server(32 << 30, n_streams=4, ws_bytes_per_stream=4 << 30, device=gpu.index or 0)
+    srv = serving.Server(32 << 30, n_streams=4, pool_bytes=16 << 30, device=gpu.index or 0)
     for jb in jobs:
         srv.register(jb.name, jb)
     mixed = requests()
@@ -136,3 +136,62 @@ def test_workspace_size_without_a_job(gpu, dtype):
             for grad in (0, 1):
                 got = int(lib.hx_job_workspace_bytes_for(job.T, job.S, job.slice, job.C, K, code, grad))
                 assert got == job.workspace_bytes(K, bool(grad)), (sp["name"], K, grad)
+
+
+@pytest.mark.parametrize("dtype,rl,rg", [(F64, 1e-12, 1e-9), (torch.float32, 1e-6, 1e-5)])
+@pytest.mark.parametrize("batched", [False, True])
+def test_served_model_request_matches_oracle(gpu, dtype, rl, rg, batched):
+    """A resident DNA job evaluated under each request's own GTR model (hx_job_eval_model),
+    with gradients for the exchangeabilities, frequencies, Weibull shape and branch lengths by
+    autograd through the served d/dQ, d/dpi, d/drates, d/dprops; against the oracle. Batched:
+    one model per sample; otherwise one model for all samples. The job is registered with
+    another model and two pattern slices, which the request's model must override."""
+    print(f"case: served per-request model {dtype} batched={batched}", flush=True)
+    from torchtree_amd import models as hm
+    from torchtree_amd import serving, synthetic
+
+    rng = np.random.default_rng(91)
+    sp = synthetic.random_job_spec("dna", 30, 900, rng)
+    post = [tuple(int(x) for x in r) for r in sp["topo"].postorder]
+    job = serving.ResidentJob("m", post, sp["codes"], sp["table"], sp["weights"], sp["q_norm"],
+                              sp["freqs"], sp["rates"], sp["props"], dtype=dtype,
+                              slice_patterns=500, device=gpu.index or 0)
+    K = 3
+    bls = synthetic.draw_bls(sp, K, 4)
+    lead = (K,) if batched else ()
+    g = torch.Generator().manual_seed(3)
+    exch0 = torch.rand(lead + (6,), generator=g, dtype=F64) * 2 + 0.3
+    fr0 = torch.rand(lead + (4,), generator=g, dtype=F64) + 0.5
+    fr0 = fr0 / fr0.sum(-1, keepdim=True)
+    shape0 = torch.tensor([0.8], dtype=F64)
+
+    def run(engine):
+        ex = exch0.clone().requires_grad_(True)
+        fr = fr0.clone().requires_grad_(True)
+        sh = shape0.clone().requires_grad_(True)
+        b = bls.clone().requires_grad_(True)
+        if engine:
+            rates, props = hm.weibull_rates(sh, 4)
+            qn = hm.normalize(hm.gtr_q(ex, fr), fr)
+            ll = serving.served_loglik(job, b.to(gpu), rates, props, qn, fr)
+        else:
+            rates, props = omodels.weibull_rates_probs(sh, 4)
+            q = omodels.gtr_q(ex, fr)
+            mats = omodels.p_t_eigh(q, fr, b.unsqueeze(-1) * rates.reshape(1, 1, -1))
+            tbl = torch.as_tensor(sp["table"])
+            tips = [tbl[torch.as_tensor(c).long()].t().contiguous() for c in sp["codes"]]
+            ll = opruning.treelik_discrete_rescaled(
+                tips + [None] * (len(tips) - 1), torch.as_tensor(sp["weights"]), post, mats,
+                fr.reshape(-1, 1, 4), props.reshape(-1, 1, 1)).reshape(-1)
+        (ll * torch.arange(1, K + 1, dtype=F64, device=ll.device)).sum().backward()
+        return ll.detach().cpu(), b.grad, ex.grad, fr.grad, sh.grad
+
+    got, ref = run(True), run(False)
+    rel = float(((got[0] - ref[0]).abs() / ref[0].abs()).max())
+    assert rel < rl, rel
+    for name, a, b in zip(("bls", "exch", "freqs", "shape"), got[1:], ref[1:]):
+        a = a.detach().cpu()
+        b2 = b[:, :-1] if name == "bls" else b
+        a2 = a[:, :-1] if name == "bls" else a
+        err = float((a2 - b2).norm() / b2.norm())
+        assert err < rg, (name, err)

@@ -86,10 +86,38 @@ def test_tip_states_shims(gpu, kind, S, dtype, rl, rg, rescaled):
     got.sum().backward()
     rel = float(((got.detach().cpu().double() - ref.detach()).abs() / ref.detach().abs()).max())
     assert rel < rl, rel
-    for name, a, b in (("mats", md.grad, m.grad), ("freqs", fd.grad, f.grad),
-                       ("props", pd.grad, p.grad)):
+    for name, a, b in (("freqs", fd.grad, f.grad), ("props", pd.grad, p.grad)):
         err = float((a.cpu().double() - b).norm() / b.norm())
         assert err < rg, (name, err)
+    # d/dmats: internal branches exactly. A tip of unknown state is, in the reference, a column
+    # of ones appended to P (tree_likelihood.py:106-112), which does not depend on P; the engine
+    # reads it as P times a row of ones, the same value (rows of P sum to 1) with a gradient
+    # along each row's all-ones direction. Every derivative through P(t) is blind to that
+    # direction (d/dt of a row sum is 0), so tip branches are compared with each row's mean
+    # removed, and the branch-length derivative through P(t) = expm(Q t) is compared in full.
+    gm, om = md.grad.cpu().double(), m.grad
+    err = float((gm[:, T:] - om[:, T:]).norm() / om[:, T:].norm())
+    assert err < rg, ("mats, internal branches", err)
+    centre = lambda x: x - x.mean(-1, keepdim=True)  # noqa: E731
+    err = float((centre(gm[:, :T]) - centre(om[:, :T])).norm() / centre(om[:, :T]).norm())
+    assert err < rg, ("mats, tip branches (row means removed)", err)
+    pi = freqs.reshape(-1)
+    q = omodels.normalize_q(omodels.q_from_exchangeabilities(
+        torch.tensor(rng.gamma(2.0, 0.5, S * (S - 1) // 2), dtype=F64), pi), pi)
+    tau = torch.tensor(rng.exponential(0.15, (2, 2 * T - 2, C)), dtype=F64)
+
+    def dtau(fn, dev, dt):
+        tt = tau.clone().to(dev).requires_grad_(True)
+        mt = torch.matrix_exp(q.to(dev)[None, None, None] * tt[..., None, None]).to(dt)
+        ll = fn([c.to(dev) for c in codes] + [None] * (T - 1), weights.to(dev), post, mt,
+                freqs.to(dev, dt), props.to(dev, dt))
+        ll.sum().backward()
+        return tt.grad.cpu()
+
+    g_ref = dtau(ofn, "cpu", F64)
+    g_got = dtau(hfn, gpu, dtype)
+    err = float((g_got - g_ref).norm() / g_ref.norm())
+    assert err < rg, ("d/dtau through P(t)", err)
 
 
 def _deep_case(T, rng):

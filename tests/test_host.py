@@ -207,14 +207,14 @@ def test_slice_for_fits_the_stream_workspace():
         assert full > lib.hx_job_workspace_bytes_for(T, S, N, C, 32, code, 0) > 0
         one = lib.hx_job_workspace_bytes_for(T, S, 1, C, 32, code, 1)
         for frac in (0.3, 0.7, 1.0):
-            srv = SimpleNamespace(ws_bytes=int(full * frac))
-            if srv.ws_bytes < one:
+            srv = SimpleNamespace(max_request_bytes=int(full * frac))
+            if srv.max_request_bytes < one:
                 with pytest.raises(serving.BudgetError):
                     serving.Server.slice_for(srv, T, N, C, S, dt)
                 continue
             sl = serving.Server.slice_for(srv, T, N, C, S, dt)
             assert 1 <= sl <= N
-            assert lib.hx_job_workspace_bytes_for(T, S, sl, C, 32, code, 1) <= srv.ws_bytes
+            assert lib.hx_job_workspace_bytes_for(T, S, sl, C, 32, code, 1) <= srv.max_request_bytes
             if frac == 1.0:
                 assert sl == N
 

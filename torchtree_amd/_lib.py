@@ -17,6 +17,7 @@ LIB_PATH = Path(__file__).resolve().parent / "lib" / "libhiptree.so"
 
 HX_F32 = 0
 HX_F64 = 1
+HX_F32_BF16X3 = 2  # FP32 partials, dense-state products as split BF16
 HX_TIPS_CODES = 0
 HX_TIPS_DENSE = 1
 
@@ -55,6 +56,13 @@ _SIGNATURES = {
     "hx_pmatrix_grad": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int,
                                  _dp, _dp, _dp, _dp, _dp, _c_int,
                                  _dp, _dp, _dp, _dp, _vp, _vp]),
+    "hx_pmatrix_expm": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int,
+                                 _dp, _dp, _dp, _c_int, _vp, _c_int, _vp]),
+    "hx_pmatrix_expm_dtau": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int,
+                                      _dp, _dp, _dp, _c_int, _vp, _c_int, _vp]),
+    "hx_pmatrix_expm_grad_scratch_bytes": (_c_size, [_c_int, _c_int, _c_int, _c_int]),
+    "hx_pmatrix_expm_grad": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int,
+                                      _dp, _dp, _dp, _c_int, _dp, _dp, _dp, _dp, _vp, _vp]),
     "hx_job_create": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int,
                                _vp, _vp, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_int,
                                ctypes.POINTER(_vp)]),
@@ -63,6 +71,9 @@ _SIGNATURES = {
     "hx_job_workspace_bytes": (_c_size, [_vp, _c_int, _c_int]),
     "hx_job_workspace_bytes_for": (_c_size, [_c_int] * 7),
     "hx_job_eval": (_c_int, [_vp, _c_int, _dp, _dp, _dp, _vp, _c_size, _vp]),
+    "hx_job_model_workspace_bytes": (_c_size, [_vp, _c_int, _c_int]),
+    "hx_job_eval_model": (_c_int, [_vp, _c_int, _dp, _c_int, _dp, _dp, _dp, _dp, _dp, _dp, _c_int,
+                                   _dp, _dp, _dp, _dp, _dp, _dp, _vp, _c_size, _vp]),
     "hx_selftest_mfma_layout": (_c_int, [ctypes.POINTER(_c_int), _vp]),
 }
 

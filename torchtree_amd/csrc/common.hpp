@@ -45,6 +45,11 @@ inline int next_pow2(int x) {
 
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+// The precisions a call may ask for (HX_F32_BF16X3 stores FP32, as HX_F32 does).
+inline bool dtype_ok(int dtype) {
+  return dtype == HX_F32 || dtype == HX_F64 || dtype == HX_F32_BF16X3;
+}
+
 // Pattern tiling shared by every pass of one call: each block owns a chunk of
 // consecutive pattern tiles for one (sample, node).
 struct Tiling {

@@ -13,6 +13,7 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <string>
 #include <vector>
 
 struct hx_job_s {
@@ -102,6 +103,60 @@ static JobWs job_ws(const hx_job_s* j, int K, int grad) {
   return job_ws_shape(j->plan->T, j->S, j->slice, j->C, K, j->dtype, grad);
 }
 
+// Workspace of one hx_job_eval_model request: P in the job's dtype, the request's frequencies
+// and proportions cast to it, one pattern slice's pruning workspace (d loglik / d mats mode),
+// that slice's d loglik / d mats in FP64 with the chain-rule scratch, and the per-slice sums
+// and their accumulators.
+struct ModelWs {
+  size_t mats, freqs, props, prune, prune_bytes, dmats, pgrad, ll, dfr, dpr, dbls, drates, dq;
+  size_t acc_dfr, acc_dpr, acc_dbls, acc_drates, acc_dq, total;
+};
+
+static ModelWs model_ws(const hx_job_s* j, int K, int eigen_batched) {
+  const size_t rb = j->dtype == HX_F64 ? 8 : 4;
+  const size_t S = j->S, C = j->C, B = j->B, E = eigen_batched ? K : 1;
+  const size_t nm = (size_t)K * B * C * S * S;
+  ModelWs w{};
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off = align_up(off + bytes, 256);
+    return at;
+  };
+  w.mats = take(nm * rb);
+  w.freqs = take((size_t)K * S * rb);
+  w.props = take((size_t)K * C * rb);
+  w.prune_bytes = workspace_bytes_for(j->plan->T, j->S, j->slice, j->C, K, j->dtype, HX_GRAD_MATS);
+  w.prune = take(w.prune_bytes);
+  w.dmats = take(nm * 8);
+  w.pgrad = take(hx_pmatrix_grad_scratch_bytes(j->S, K, j->B, j->C));
+  w.ll = take((size_t)K * 8);
+  w.dfr = take((size_t)K * S * 8);
+  w.dpr = take((size_t)K * C * 8);
+  w.dbls = take((size_t)K * B * 8);
+  w.drates = take((size_t)K * C * 8);
+  w.dq = take(E * S * S * 8);
+  w.acc_dfr = take((size_t)K * S * 8);
+  w.acc_dpr = take((size_t)K * C * 8);
+  w.acc_dbls = take((size_t)K * B * 8);
+  w.acc_drates = take((size_t)K * C * 8);
+  w.acc_dq = take(E * S * S * 8);
+  w.total = off;
+  return w;
+}
+
+template <typename R>
+__global__ void k_cast_rows(const double* __restrict__ src, R* __restrict__ dst, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (R)src[i];
+}
+
+static void cast_rows(const double* src, void* dst, size_t n, int dtype, hipStream_t st) {
+  const unsigned g = (unsigned)((n + 255) / 256);
+  if (dtype == HX_F64) k_cast_rows<double><<<g, 256, 0, st>>>(src, static_cast<double*>(dst), n);
+  else k_cast_rows<float><<<g, 256, 0, st>>>(src, static_cast<float*>(dst), n);
+}
+
 }  // namespace hx
 
 extern "C" {
@@ -116,7 +171,7 @@ int hx_job_create(hx_plan_t plan, int S, int N, int C, int dtype, int slice_patt
              HX_ERR_INVALID_ARGUMENT, "null argument");
   HX_REQUIRE(S >= 2 && S <= 64 && N >= 1 && C >= 1 && ncodes >= 1 && ncodes <= hx::kMaxCodes,
              HX_ERR_INVALID_ARGUMENT, "bad dimensions");
-  HX_REQUIRE(dtype == HX_F32 || dtype == HX_F64, HX_ERR_INVALID_ARGUMENT, "bad dtype");
+  HX_REQUIRE(hx::dtype_ok(dtype), HX_ERR_INVALID_ARGUMENT, "bad dtype");
   // The kernels index the code table with the raw code, so every code must name a row.
   const uint8_t max_code = *std::max_element(codes, codes + (size_t)plan->T * N);
   HX_REQUIRE(max_code < ncodes, HX_ERR_INVALID_ARGUMENT, "tip code outside the code table");
@@ -230,17 +285,18 @@ size_t hx_job_workspace_bytes(hx_job_t job, int K, int want_grad) {
 size_t hx_job_workspace_bytes_for(int taxa, int S, int slice_patterns, int C, int K, int dtype,
                                   int want_grad) {
   if (taxa < 2 || S < 2 || slice_patterns < 1 || C < 1 || K < 1) return 0;
-  if (dtype != HX_F32 && dtype != HX_F64) return 0;
+  if (!hx::dtype_ok(dtype)) return 0;
   return hx::job_ws_shape(taxa, S, slice_patterns, C, K, dtype, want_grad).total;
 }
 
-int hx_job_eval(hx_job_t job, int K, const double* bls, double* out_loglik, double* out_dbls,
-                void* workspace, size_t workspace_bytes, hx_stream_t stream) {
-  HX_REQUIRE(job && bls && out_loglik && workspace && K >= 1, HX_ERR_INVALID_ARGUMENT,
-             "null argument");
+}  // extern "C"
+
+namespace hx {
+
+// One request of hx_job_eval in a workspace laid out by job_ws.
+static int job_eval_in(hx_job_t job, int K, const double* bls, double* out_loglik,
+                       double* out_dbls, const JobWs& w, void* workspace, hx_stream_t stream) {
   const int grad = out_dbls != nullptr;
-  const hx::JobWs w = hx::job_ws(job, K, grad);
-  HX_REQUIRE(workspace_bytes >= w.total, HX_ERR_WORKSPACE_TOO_SMALL, "workspace too small");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   char* base = static_cast<char*>(workspace);
   void* mats = base + w.mats;
@@ -283,6 +339,106 @@ int hx_job_eval(hx_job_t job, int K, const double* bls, double* out_loglik, doub
   }
   HX_HIP_TRY(hipGetLastError());
   return HX_OK;
+}
+
+struct ModelArgs {
+  int eigen_batched;
+  const double *lambda, *left, *right, *rates, *props, *freqs;
+  int model_batched;
+  double *out_loglik, *out_dbls, *out_drates, *out_dq, *out_dfreqs, *out_dprops;
+};
+
+// One request with its own eigensystem, category rates, proportions and frequencies. Per
+// pattern slice: post-order (logL, d/dfreqs and d/dprops at the root), pre-order d/dmats,
+// and the chain rule to d/dbls, d/drates and d/dQ, all linear in the slice's terms, so the
+// slices' FP64 results are summed in slice order.
+static int job_eval_model_in(hx_job_t job, int K, const double* bls, const ModelArgs& a,
+                             const ModelWs& w, void* workspace, hx_stream_t stream) {
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  char* base = static_cast<char*>(workspace);
+  auto at = [&](size_t off) { return reinterpret_cast<double*>(base + off); };
+  const int S = job->S, C = job->C, B = job->B;
+  const int E = a.eigen_batched ? K : 1;
+  const int mb = a.model_batched && K > 1;
+  const int rows = mb ? K : 1;
+  void* mats = base + w.mats;
+  int rc = hx_pmatrix(S, K, B, C, a.eigen_batched, a.lambda, a.left, a.right, bls, a.rates, mb,
+                      mats, job->dtype, stream);
+  if (rc != HX_OK) return rc;
+  cast_rows(a.freqs, base + w.freqs, (size_t)rows * S, job->dtype, st);
+  cast_rows(a.props, base + w.props, (size_t)rows * C, job->dtype, st);
+  const size_t nkb = (size_t)K * B, nks = (size_t)K * S, nkc = (size_t)K * C;
+  const size_t nq = (size_t)E * S * S;
+  fill0(a.out_loglik, K, st);
+  fill0(at(w.acc_dfr), nks, st);
+  fill0(at(w.acc_dpr), nkc, st);
+  fill0(at(w.acc_dbls), nkb, st);
+  fill0(at(w.acc_drates), nkc, st);
+  fill0(at(w.acc_dq), nq, st);
+  for (int s = 0; s < job->nslices; ++s) {
+    const int n = job->slice_n[s];
+    rc = hx_loglik(job->plan, S, n, C, K, job->dtype, HX_TIPS_CODES, job->d_codes[s],
+                   job->d_table, job->ncodes, job->d_weights[s], mats, base + w.freqs, mb,
+                   base + w.props, mb, base + w.prune, w.prune_bytes, at(w.ll), at(w.dfr),
+                   at(w.dpr), stream);
+    if (rc != HX_OK) return rc;
+    rc = hx_loglik_grad_mats(job->plan, S, n, C, K, job->dtype, HX_TIPS_CODES, job->d_codes[s],
+                             job->d_table, job->ncodes, job->d_weights[s], mats, base + w.freqs,
+                             mb, base + w.props, mb, base + w.prune, w.prune_bytes, at(w.dmats),
+                             stream);
+    if (rc != HX_OK) return rc;
+    rc = hx_pmatrix_grad(S, K, B, C, a.eigen_batched, a.lambda, a.left, a.right, bls, a.rates, mb,
+                         at(w.dmats), at(w.dbls), at(w.drates), at(w.dq), base + w.pgrad, stream);
+    if (rc != HX_OK) return rc;
+    accum(a.out_loglik, at(w.ll), K, st);
+    accum(at(w.acc_dfr), at(w.dfr), nks, st);
+    accum(at(w.acc_dpr), at(w.dpr), nkc, st);
+    accum(at(w.acc_dbls), at(w.dbls), nkb, st);
+    accum(at(w.acc_drates), at(w.drates), nkc, st);
+    accum(at(w.acc_dq), at(w.dq), nq, st);
+  }
+  struct Out { double* dst; size_t src, n; };
+  const Out outs[] = {{a.out_dbls, w.acc_dbls, nkb}, {a.out_drates, w.acc_drates, nkc},
+                      {a.out_dq, w.acc_dq, nq}, {a.out_dfreqs, w.acc_dfr, nks},
+                      {a.out_dprops, w.acc_dpr, nkc}};
+  for (const Out& o : outs)
+    if (o.dst) HX_HIP_TRY(hipMemcpyAsync(o.dst, at(o.src), o.n * 8, hipMemcpyDeviceToDevice, st));
+  HX_HIP_TRY(hipGetLastError());
+  return HX_OK;
+}
+
+}  // namespace hx
+
+extern "C" {
+
+int hx_job_eval(hx_job_t job, int K, const double* bls, double* out_loglik, double* out_dbls,
+                void* workspace, size_t workspace_bytes, hx_stream_t stream) {
+  HX_REQUIRE(job && bls && out_loglik && workspace && K >= 1, HX_ERR_INVALID_ARGUMENT,
+             "null argument");
+  const hx::JobWs w = hx::job_ws(job, K, out_dbls != nullptr);
+  HX_REQUIRE(workspace_bytes >= w.total, HX_ERR_WORKSPACE_TOO_SMALL, "workspace too small");
+  return hx::job_eval_in(job, K, bls, out_loglik, out_dbls, w, workspace, stream);
+}
+
+size_t hx_job_model_workspace_bytes(hx_job_t job, int K, int eigen_batched) {
+  if (!job || K < 1) return 0;
+  return hx::model_ws(job, K, eigen_batched).total;
+}
+
+int hx_job_eval_model(hx_job_t job, int K, const double* bls, int eigen_batched,
+                      const double* lambda, const double* left, const double* right,
+                      const double* rates, const double* props, const double* freqs,
+                      int model_batched, double* out_loglik, double* out_dbls, double* out_drates,
+                      double* out_dq, double* out_dfreqs, double* out_dprops, void* workspace,
+                      size_t workspace_bytes, hx_stream_t stream) {
+  HX_REQUIRE(job && bls && lambda && left && right && rates && props && freqs && out_loglik &&
+                 workspace && K >= 1,
+             HX_ERR_INVALID_ARGUMENT, "null argument");
+  const hx::ModelWs w = hx::model_ws(job, K, eigen_batched);
+  HX_REQUIRE(workspace_bytes >= w.total, HX_ERR_WORKSPACE_TOO_SMALL, "workspace too small");
+  const hx::ModelArgs a{eigen_batched, lambda, left, right, rates, props, freqs, model_batched,
+                        out_loglik, out_dbls, out_drates, out_dq, out_dfreqs, out_dprops};
+  return hx::job_eval_model_in(job, K, bls, a, w, workspace, stream);
 }
 
 }  // extern "C"

@@ -270,7 +270,7 @@ static int pmatrix_impl(int S, int K, int B, int C, int eigen_batched, const dou
              "bad dimensions (S must be 2..64)");
   HX_REQUIRE(lambda && left && right && bls && rates && mats, HX_ERR_INVALID_ARGUMENT,
              "null pointer argument");
-  HX_REQUIRE(dtype == HX_F32 || dtype == HX_F64, HX_ERR_INVALID_ARGUMENT, "bad dtype");
+  HX_REQUIRE(hx::dtype_ok(dtype), HX_ERR_INVALID_ARGUMENT, "bad dtype");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   const size_t nmat = (size_t)K * B * C;
   if (S <= 8) {

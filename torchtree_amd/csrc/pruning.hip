@@ -102,16 +102,32 @@ __global__ void k_reduce_tau(const double* __restrict__ gtau, double* __restrict
 
 // P and P^T of every (sample, branch, category) into zero-padded 64 x 64 tiles:
 // ppad[kbc][0][i][j] = P[i][j], ppad[kbc][1][j][i] = P[i][j]. One block per matrix.
+// With `split` (FP32 only) each element is stored as the 32-bit word holding its BF16 hi
+// (low half) and lo (high half), the operand form of the split-precision products.
+__device__ __forceinline__ uint32_t pad_pair(float x) {
+  const __bf16 hi = (__bf16)x;
+  const __bf16 lo = (__bf16)(x - (float)hi);
+  return (uint32_t)__builtin_bit_cast(uint16_t, hi) |
+         ((uint32_t)__builtin_bit_cast(uint16_t, lo) << 16);
+}
 template <typename R>
 __global__ __launch_bounds__(256) void k_pad_mats(const R* __restrict__ mats, R* __restrict__ ppad,
-                                                  int S) {
+                                                  int S, int split) {
   const size_t m = blockIdx.x;
   const R* src = mats + m * S * S;
   R* dst = ppad + m * 2 * 64 * 64;
   for (int e = threadIdx.x; e < 64 * 64; e += 256) {
     const int i = e >> 6, j = e & 63;
-    dst[e] = (i < S && j < S) ? src[i * S + j] : R(0);
-    dst[64 * 64 + e] = (i < S && j < S) ? src[j * S + i] : R(0);
+    R p = (i < S && j < S) ? src[i * S + j] : R(0);
+    R pt = (i < S && j < S) ? src[j * S + i] : R(0);
+    if constexpr (sizeof(R) == 4) {
+      if (split) {
+        p = __builtin_bit_cast(float, pad_pair(p));
+        pt = __builtin_bit_cast(float, pad_pair(pt));
+      }
+    }
+    dst[e] = p;
+    dst[64 * 64 + e] = pt;
   }
 }
 
@@ -122,7 +138,7 @@ struct WsLayout {
 
 // FP32 gradients at 33..64 states read P and P^T from zero-padded 64 x 64 copies.
 static bool use_ppad(int S, int dtype, int grad_mode) {
-  return grad_mode != 0 && dtype == HX_F32 && S > 32 && S <= 64;
+  return grad_mode != 0 && dtype != HX_F64 && S > 32 && S <= 64;
 }
 
 // grad_mode: 0 loglik only, 1 d loglik / d mats, 2 d loglik / d tau.
@@ -200,7 +216,7 @@ static int validate(const CallArgs& c) {
     HX_REQUIRE(2 * c.C * (c.S <= 32 ? 32 : 64) <= 512, HX_ERR_UNSUPPORTED,
                "categories x padded states exceed the dense-state kernels (C*S_pad <= 256)");
   }
-  HX_REQUIRE(c.dtype == HX_F32 || c.dtype == HX_F64, HX_ERR_INVALID_ARGUMENT, "bad dtype");
+  HX_REQUIRE(dtype_ok(c.dtype), HX_ERR_INVALID_ARGUMENT, "bad dtype");
   HX_REQUIRE(c.tip_kind == HX_TIPS_CODES || c.tip_kind == HX_TIPS_DENSE, HX_ERR_INVALID_ARGUMENT,
              "bad tip kind");
   if (c.tip_kind == HX_TIPS_CODES)
@@ -247,6 +263,7 @@ static PassArgs<R> make_args(const CallArgs& c) {
   a.mats_dt = c.grad_mode == 2 ? static_cast<const R*>(c.mats_dt) : nullptr;
   a.gtau = reinterpret_cast<double*>(base + w.gpart);
   a.ppad = use_ppad(c.S, c.dtype, c.grad_mode) ? reinterpret_cast<R*>(base + w.ppad) : nullptr;
+  a.split = c.dtype == HX_F32_BF16X3;
   return a;
 }
 
@@ -289,7 +306,8 @@ static int run_pre(const CallArgs& c, double* out, hipStream_t st) {
   const hx_plan_s* plan = c.plan;
   const int D = (int)plan->pre_off.size() - 1;
   if (a.ppad != nullptr) {
-    k_pad_mats<R><<<(unsigned)((size_t)c.K * a.B * c.C), 256, 0, st>>>(a.mats, a.ppad, c.S);
+    k_pad_mats<R><<<(unsigned)((size_t)c.K * a.B * c.C), 256, 0, st>>>(a.mats, a.ppad, c.S,
+                                                                         a.split);
     HX_HIP_TRY(hipGetLastError());
   }
   if constexpr (std::is_same<R, float>::value) {

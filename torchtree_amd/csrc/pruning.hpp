@@ -28,7 +28,9 @@ struct PassArgs {
   R* gpart;               // [B][K][nchunks][C][S][S]     (d loglik / d mats mode)
   const R* mats_dt;       // [K][B][C][S][S] dP/dtau; non-null selects d loglik / d tau mode
   double* gtau;           // [B][K][nchunks][C]            (d loglik / d tau mode)
-  R* ppad;                // [K][B][C][2][64][64]: P and P^T zero-padded (FP32, 33..64 states)
+  R* ppad;                // [K][B][C][2][64][64]: P and P^T zero-padded (FP32, 33..64 states);
+                          // with `split`, each element holds its BF16 hi and lo halves
+  int split;              // dense-state products as split BF16 on the matrix cores (FP32 only)
 };
 
 constexpr double kLn2 = 0.69314718055994530942;

@@ -45,6 +45,88 @@ __device__ __forceinline__ vec4_t<double> mma(double a, double b, vec4_t<double>
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// ---- split-precision products on the BF16 matrix cores
+//
+// A float x is carried as hi = bf16(x) and lo = bf16(x - hi) (both rounded to nearest even;
+// x - hi is exact in FP32), which keeps 16 significant bits: |x - hi - lo| <= 2^-17 |x|.
+// A product of two such operands is formed as lo_a hi_b + hi_a lo_b + hi_a hi_b on
+// v_mfma_f32_16x16x32_bf16 (8 operand values per lane, K = 32) or v_mfma_f32_16x16x16_bf16
+// (4 per lane, K = 16), accumulating in FP32; the dropped lo_a lo_b term is below 2^-16 of
+// the product. Operand and accumulator lanes follow the FP32 instruction's: A[i][k] in lane
+// i + 16 (k / n), element k % n, B[k][j] in lane j + 16 (k / n), and the FP32 16x16
+// accumulator (checked on the device by hx_selftest_mfma_layout). Every caller builds A and
+// B with the same map from (lane group, element) to the contraction index, so the sum runs
+// over the same index pairs as the FP32 products.
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
+template <int N>
+struct BfPack;
+template <>
+struct BfPack<8> {
+  bf16x8_t hi, lo;
+};
+template <>
+struct BfPack<4> {
+  bf16x4_t hi, lo;
+};
+
+// hi and lo halves of x, as one 32-bit word (hi in the low 16 bits): the form kept in LDS and
+// in the padded copies of P, so that an operand is assembled without converting again
+__device__ __forceinline__ uint32_t bf_pair(float x) {
+  const __bf16 hi = (__bf16)x;
+  const __bf16 lo = (__bf16)(x - (float)hi);
+  return (uint32_t)__builtin_bit_cast(uint16_t, hi) |
+         ((uint32_t)__builtin_bit_cast(uint16_t, lo) << 16);
+}
+__device__ __forceinline__ __bf16 pair_hi(uint32_t p) {
+  return __builtin_bit_cast(__bf16, (uint16_t)(p & 0xffffu));
+}
+__device__ __forceinline__ __bf16 pair_lo(uint32_t p) {
+  return __builtin_bit_cast(__bf16, (uint16_t)(p >> 16));
+}
+
+// Four hi/lo words from 16 aligned bytes. The whole vector is converted, then indexed:
+// a bit cast of one element of a vector lvalue reads element 0 whatever the index
+// (hipcc turns __builtin_bit_cast(uint32_t, v[e]) into a read of v's first word).
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ u32x4_t load_pairs4(const float* p) {
+  return __builtin_bit_cast(u32x4_t, *reinterpret_cast<const vec4_t<float>*>(p));
+}
+
+template <int N>
+__device__ __forceinline__ void pack_set(BfPack<N>& b, int e, float x) {
+  const __bf16 hi = (__bf16)x;
+  b.hi[e] = hi;
+  b.lo[e] = (__bf16)(x - (float)hi);
+}
+template <int N>
+__device__ __forceinline__ void pack_set_pair(BfPack<N>& b, int e, uint32_t p) {
+  b.hi[e] = pair_hi(p);
+  b.lo[e] = pair_lo(p);
+}
+
+__device__ __forceinline__ vec4_t<float> mma3(const BfPack<8>& a, const BfPack<8>& b,
+                                              vec4_t<float> c) {
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.lo, b.hi, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.lo, c, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.hi, c, 0, 0, 0);
+}
+__device__ __forceinline__ vec4_t<float> mma3(const BfPack<4>& a, const BfPack<4>& b,
+                                              vec4_t<float> c) {
+  c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a.lo, b.hi, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a.hi, b.lo, c, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a.hi, b.hi, c, 0, 0, 0);
+}
+
+// The state contractions (forward, transpose) take KV values per lane; in the split form
+// they go as packs of KB = 8 (one K = 32 instruction covers four lane groups of 8 states)
+// or KB = 4 for KV = 4, NB packs per lane, values past KV zero.
+template <int KV>
+struct SplitShape {
+  static constexpr int KB = KV <= 4 ? 4 : 8;
+  static constexpr int NB = (KV + KB - 1) / KB;
+};
+
 __device__ __forceinline__ float fmax_(float a, float b) { return fmaxf(a, b); }
 __device__ __forceinline__ double fmax_(double a, double b) { return fmax(a, b); }
 __device__ __forceinline__ float frexp_(float m, int* e) { return frexpf(m, e); }
@@ -159,12 +241,54 @@ __device__ __forceinline__ void child_row(const PassArgs<R>& a, int ch, int k, i
   }
 }
 
+// The forward product's P operand, held in registers for a chunk, and the product
+// acc[t] += L P^T over the lane's KV states g KV .. g KV + KV - 1:
+//   plain: f[t][ks] = P[16 t + nn][g KV + ks], KV steps of the 16x16x4 instruction;
+//   split: the same values as BF16 hi/lo packs b[t][ks / KB] (element ks % KB, zero past
+//          KV), NB split products per output tile.
+template <typename R, int MT, int KV, bool SPLIT>
+struct FwdP {
+  static_assert(!SPLIT || sizeof(R) == 4, "split products take FP32 operands");
+  static constexpr int KB = SplitShape<KV>::KB, NB = SplitShape<KV>::NB;
+  R f[SPLIT ? 1 : MT][SPLIT ? 1 : KV];
+  BfPack<KB> b[SPLIT ? MT : 1][SPLIT ? NB : 1];
+
+  __device__ __forceinline__ void load(const R* pm, int S, int nn, int g) {
+#pragma unroll
+    for (int t = 0; t < MT; ++t)
+#pragma unroll
+      for (int q = 0; q < (SPLIT ? NB * KB : KV); ++q) {
+        const int i = 16 * t + nn, j = g * KV + q;
+        const R v = (q < KV && i < S && j < S) ? pm[i * S + j] : R(0);
+        if constexpr (SPLIT) pack_set(b[t][q / KB], q % KB, v);
+        else f[t][q] = v;
+      }
+  }
+
+  __device__ __forceinline__ void product(const R (&arow)[KV], vec4_t<R> (&acc)[MT]) const {
+    if constexpr (SPLIT) {
+      BfPack<KB> ap[NB];
+#pragma unroll
+      for (int q = 0; q < NB * KB; ++q) pack_set(ap[q / KB], q % KB, q < KV ? arow[q] : 0.f);
+#pragma unroll
+      for (int bb = 0; bb < NB; ++bb)
+#pragma unroll
+        for (int t = 0; t < MT; ++t) acc[t] = mma3(ap[bb], b[t][bb], acc[t]);
+    } else {
+#pragma unroll
+      for (int ks = 0; ks < KV; ++ks)
+#pragma unroll
+        for (int t = 0; t < MT; ++t) acc[t] = mma(arow[ks], f[t][ks], acc[t]);
+    }
+  }
+};
+
 }  // namespace
 
 // ---------------------------------------------------------------- post-order
 // Launched for the root (ROOT = true: the per-chunk root sums of loglik, d/dfreqs
 // and d/dprops); the levels below use k_post_mfma_r.
-template <typename R, int MT, int KV, int TIPS, bool ROOT>
+template <typename R, int MT, int KV, int TIPS, bool ROOT, bool SPLIT = false>
 __global__ __launch_bounds__(512) void k_post_mfma(PassArgs<R> a,
                                                    const int32_t* __restrict__ nodes, int S) {
   constexpr int SP = 16 * MT;
@@ -184,17 +308,8 @@ __global__ __launch_bounds__(512) void k_post_mfma(PassArgs<R> a,
   const int ch0 = a.kids[2 * v], ch1 = a.kids[2 * v + 1];
   const int chh = h ? ch1 : ch0;
 
-  R Pf[MT][KV];  // Pf[t][ks] = P[16 t + nn][g KV + ks]
-  {
-    const R* pm = a.mats + (((size_t)k * a.B + chh) * C + c) * S * S;
-#pragma unroll
-    for (int t = 0; t < MT; ++t)
-#pragma unroll
-      for (int ks = 0; ks < KV; ++ks) {
-        const int i = 16 * t + nn, j = g * KV + ks;
-        Pf[t][ks] = (i < S && j < S) ? pm[i * S + j] : R(0);
-      }
-  }
+  FwdP<R, MT, KV, SPLIT> fp;  // P[16 t + nn][g KV + ks]
+  fp.load(a.mats + (((size_t)k * a.B + chh) * C + c) * S * S, S, nn, g);
   R pi_t[MT];
   R pc = R(0);
   double acc_l = 0.0, acc_p = 0.0, acc_f[MT];
@@ -223,11 +338,7 @@ __global__ __launch_bounds__(512) void k_post_mfma(PassArgs<R> a,
     for (int t = 0; t < MT; ++t) acc[t] = vec4_t<R>{R(0), R(0), R(0), R(0)};
     R arow[KV];
     child_row<R, TIPS, KV>(a, chh, k, nbase + nn, c, g * KV, S, arow);
-#pragma unroll
-    for (int ks = 0; ks < KV; ++ks) {
-#pragma unroll
-      for (int t = 0; t < MT; ++t) acc[t] = mma(arow[ks], Pf[t][ks], acc[t]);
-    }
+    fp.product(arow, acc);
     // lane (nn, g): acc[t][r] = Y[m = 4 g + r][i = 16 t + nn]
     if (h == 1) {
 #pragma unroll
@@ -339,7 +450,7 @@ __global__ __launch_bounds__(512) void k_post_mfma(PassArgs<R> a,
 // double-buffered by tile parity so two barriers per tile suffice), and a tip
 // child's codes for the chunk are staged in LDS once. Same products as
 // k_post_mfma.
-template <typename R, int MT, int KV, int TIPS>
+template <typename R, int MT, int KV, int TIPS, bool SPLIT = false>
 __global__ __launch_bounds__(512) void k_post_mfma_r(PassArgs<R> a,
                                                      const int32_t* __restrict__ nodes, int S) {
   constexpr int SP = 16 * MT;
@@ -376,17 +487,8 @@ __global__ __launch_bounds__(512) void k_post_mfma_r(PassArgs<R> a,
   // categories C .. 3 of mx stay zero, so a row's maximum reads all four entries
   for (int q = threadIdx.x; q < 4 * kTile; q += blockDim.x) mx[q] = R(0);
 
-  R Pf[MT][KV];  // Pf[t][ks] = P[16 t + nn][g KV + ks]
-  {
-    const R* pm = a.mats + (((size_t)k * a.B + chh) * C + c) * S * S;
-#pragma unroll
-    for (int t = 0; t < MT; ++t)
-#pragma unroll
-      for (int ks = 0; ks < KV; ++ks) {
-        const int i = 16 * t + nn, j = g * KV + ks;
-        Pf[t][ks] = (i < S && j < S) ? pm[i * S + j] : R(0);
-      }
-  }
+  FwdP<R, MT, KV, SPLIT> fp;  // P[16 t + nn][g KV + ks]
+  fp.load(a.mats + (((size_t)k * a.B + chh) * C + c) * S * S, S, nn, g);
   bool stok[MT];
 #pragma unroll
   for (int t = 0; t < MT; ++t) stok[t] = 16 * t + nn < S;
@@ -427,10 +529,7 @@ __global__ __launch_bounds__(512) void k_post_mfma_r(PassArgs<R> a,
     vec4_t<R> acc[MT];
 #pragma unroll
     for (int t = 0; t < MT; ++t) acc[t] = vec4_t<R>{R(0), R(0), R(0), R(0)};
-#pragma unroll
-    for (int ks = 0; ks < KV; ++ks)
-#pragma unroll
-      for (int t = 0; t < MT; ++t) acc[t] = mma(arow[ks], Pf[t][ks], acc[t]);
+    fp.product(arow, acc);
     // lane (nn, g): acc[t][r] = Y[m = acc_row(g, r)][i = 16 t + nn]
     if (h == 1) {
 #pragma unroll
@@ -480,10 +579,11 @@ __global__ __launch_bounds__(512) void k_post_mfma_r(PassArgs<R> a,
 }
 
 // ----------------------------------------------------------------- pre-order
-template <typename R, int MT, int KV, int TIPS, bool ROOT>
+template <typename R, int MT, int KV, int TIPS, bool ROOT, bool SPLIT = false>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
     sizeof(R) == 4 && MT <= 2 ? 4 : 1))) void k_pre_mfma(PassArgs<R> a,
                                                   const int32_t* __restrict__ nodes, int S) {
+  static_assert(!SPLIT, "k_pre_mfma serves FP64 at 64 padded states only");
   constexpr int SP = 16 * MT;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int C = a.C;
@@ -743,11 +843,15 @@ __device__ __forceinline__ void pre_rows(const PassArgs<R>& a, bool internal, co
 // Addresses are a (node, sample) slab base plus 32-bit element offsets,
 // (n C + c) S + s. A tip child's codes for the chunk are staged in LDS once.
 // Same products and barriers as k_pre_mfma.
-template <typename R, int MT, int KV, int TIPS, bool ROOT>
+template <typename R, int MT, int KV, int TIPS, bool ROOT, bool SPLIT = false>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
-    sizeof(R) == 4 && MT <= 2 ? 4 : 1))) void k_pre_mfma_r(PassArgs<R> a,
-                                                           const int32_t* __restrict__ nodes, int S) {
+    sizeof(R) == 4 && MT <= 2 ? (SPLIT && KV == 8 ? 3 : 4) : 1))) void k_pre_mfma_r(
+    PassArgs<R> a, const int32_t* __restrict__ nodes, int S) {
   static_assert(MT <= 2 || (sizeof(R) == 4 && KV == 16), "64 padded states: FP32, 16 steps");
+  // split products: the statistic's L tile, the transpose's P tile and the padded P copies
+  // hold each value's BF16 hi/lo pair in its 32-bit slot
+  using SSh = SplitShape<KV>;
+  constexpr int KB = SSh::KB, NB = SSh::NB;
   constexpr bool kPReg = MT <= 2;
   constexpr int SP = 16 * MT;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -790,17 +894,18 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
   // Pf[t][ks] = P[16 t + nn][g KV + ks] (forward), Pt[t][ks] = P[g KV + ks][16 t + nn]
   // (transpose)
   // the transpose's operand P[g KV + ks][16 t + nn] lives in this wave's LDS tile
-  R Pf[kPReg ? MT : 1][kPReg ? KV : 1];
+  FwdP<R, kPReg ? MT : 1, kPReg ? KV : 1, SPLIT && kPReg> fp;
   R* ptw = PtL + (size_t)(h * C + c) * 4 * KV * kPtS;
   if constexpr (kPReg) {
+    fp.load(pm, S, nn, g);
 #pragma unroll
     for (int t = 0; t < MT; ++t)
 #pragma unroll
       for (int ks = 0; ks < KV; ++ks) {
         const int i = 16 * t + nn, j = g * KV + ks;
-        const bool ok = i < S && j < S;
-        Pf[t][ks] = ok ? pm[i * S + j] : R(0);
-        ptw[j * kPtS + i] = ok ? pm[j * S + i] : R(0);
+        const R v = (i < S && j < S) ? pm[j * S + i] : R(0);
+        if constexpr (SPLIT) ptw[j * kPtS + i] = __builtin_bit_cast(float, bf_pair(v));
+        else ptw[j * kPtS + i] = v;
       }
   }
   // padded copies: [0][i][j] = P[i][j], [1][j][i] = P[i][j]; lane (nn, g) reads row
@@ -874,12 +979,30 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
     for (int t = 0; t < MT; ++t) acc[t] = vec4_t<R>{R(0), R(0), R(0), R(0)};
 #pragma unroll
-    for (int q = 0; q < KV; ++q) lt[nn * SP + g * KV + q] = arow[q];
+    for (int q = 0; q < KV; ++q) {
+      if constexpr (SPLIT) lt[nn * SP + g * KV + q] = __builtin_bit_cast(float, bf_pair(arow[q]));
+      else lt[nn * SP + g * KV + q] = arow[q];
+    }
     if constexpr (kPReg) {
+      fp.product(arow, acc);
+    } else if constexpr (SPLIT) {
+      // 64 padded states: KV = 16 = two packs of 8; ppf holds the hi/lo pairs
+      BfPack<8> ap[2];
 #pragma unroll
-      for (int ks = 0; ks < KV; ++ks)
+      for (int q = 0; q < 16; ++q) pack_set(ap[q / 8], q % 8, arow[q]);
 #pragma unroll
-        for (int t = 0; t < MT; ++t) acc[t] = mma(arow[ks], Pf[t][ks], acc[t]);
+      for (int t = 0; t < MT; ++t)
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb) {  // one pack of P live at a time
+          BfPack<8> bp;
+#pragma unroll
+          for (int q = 0; q < 8; q += 4) {
+            const u32x4_t pv = load_pairs4(ppf + 16 * t * 64 + 8 * bb + q);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) pack_set_pair(bp, q + e, pv[e]);
+          }
+          acc[t] = mma3(ap[bb], bp, acc[t]);
+        }
     } else {
 #pragma unroll
       for (int t = 0; t < MT; ++t)
@@ -930,16 +1053,36 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
       const R den = (dv[0] + dv[1]) + (dv[2] + dv[3]);
       coef[r] = rok[r] ? div_(wt[r], den) * pc : R(0);
     }
+    if constexpr (SPLIT) {
+      // one K = 16 split product per output tile: lane group g holds the patterns
+      // acc_row(g, 0..3) = 4 g .. 4 g + 3 as elements 0..3 of A and B
+      BfPack<4> bp[MT];
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
-      R bv[MT];
+      for (int nt = 0; nt < MT; ++nt)
 #pragma unroll
-      for (int nt = 0; nt < MT; ++nt) bv[nt] = lt[acc_row<R>(g, ks) * SP + 16 * nt + nn];
+        for (int ks = 0; ks < 4; ++ks)
+          pack_set_pair(bp[nt], ks, __builtin_bit_cast(
+                                        uint32_t, lt[acc_row<R>(g, ks) * SP + 16 * nt + nn]));
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) {
-        const R av = coef[ks] * u[mt][ks];
+        BfPack<4> ap;
 #pragma unroll
-        for (int nt = 0; nt < MT; ++nt) G[mt][nt] = mma(av, bv[nt], G[mt][nt]);
+        for (int ks = 0; ks < 4; ++ks) pack_set(ap, ks, coef[ks] * u[mt][ks]);
+#pragma unroll
+        for (int nt = 0; nt < MT; ++nt) G[mt][nt] = mma3(ap, bp[nt], G[mt][nt]);
+      }
+    } else {
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        R bv[MT];
+#pragma unroll
+        for (int nt = 0; nt < MT; ++nt) bv[nt] = lt[acc_row<R>(g, ks) * SP + 16 * nt + nn];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+          const R av = coef[ks] * u[mt][ks];
+#pragma unroll
+          for (int nt = 0; nt < MT; ++nt) G[mt][nt] = mma(av, bv[nt], G[mt][nt]);
+        }
       }
     }
     // transpose Ut through this wave's own LDS rows (only this wave reads them)
@@ -951,7 +1094,38 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
     if (internal) {
 #pragma unroll
       for (int t = 0; t < MT; ++t) nu[t] = vec4_t<R>{R(0), R(0), R(0), R(0)};
-      if constexpr (kPReg) {
+      if constexpr (SPLIT) {
+        // K index (pack q / KB, element q % KB, lane group g) = state g KV + q, as in the
+        // forward product; the P operand comes as hi/lo pairs from the LDS tile (KV <= 8)
+        // or the padded copies (KV = 16)
+        BfPack<KB> ap[NB];
+#pragma unroll
+        for (int q = 0; q < NB * KB; ++q)
+          pack_set(ap[q / KB], q % KB, q < KV ? mine[nn * SP + g * KV + q] : 0.f);
+#pragma unroll
+        for (int t = 0; t < MT; ++t)
+#pragma unroll
+          for (int bb = 0; bb < NB; ++bb) {  // one pack of P live at a time
+            BfPack<KB> bp;
+            if constexpr (kPReg) {
+#pragma unroll
+              for (int e = 0; e < KB; ++e) {
+                const int q = bb * KB + e;
+                pack_set_pair(bp, e, q < KV ? __builtin_bit_cast(
+                                                  uint32_t, ptw[(g * KV + q) * kPtS + 16 * t + nn])
+                                            : 0u);
+              }
+            } else {
+#pragma unroll
+              for (int q = 0; q < KB; q += 4) {
+                const u32x4_t pv = load_pairs4(ppt + 16 * t * 64 + bb * KB + q);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) pack_set_pair(bp, q + e, pv[e]);
+              }
+            }
+            nu[t] = mma3(ap[bb], bp, nu[t]);
+          }
+      } else if constexpr (kPReg) {
 #pragma unroll
         for (int ks = 0; ks < KV; ++ks) {
           const R av = mine[nn * SP + g * KV + ks];  // K index (ks, g) = state g KV + ks
@@ -1039,9 +1213,9 @@ static size_t mfma_lds(int C, int S, bool pre, int tiles_per_chunk) {
   return (C * kTile * SP + C * kTile) * rb + (size_t)(C * kTile + C * SP) * 8 + 3 * kTile * 4 + 64;
 }
 
-template <typename R>
-int launch_mfma(bool post, const PassArgs<R>& a, const int32_t* nodes, int width, int tips,
-                bool root, int S, hipStream_t st) {
+template <typename R, bool SPLIT>
+static int launch_mfma_t(bool post, const PassArgs<R>& a, const int32_t* nodes, int width,
+                         int tips, bool root, int S, hipStream_t st) {
   const int MT = S <= 16 ? 1 : (S <= 32 ? 2 : 4);
   const int threads = 2 * a.C * 64;
   const size_t lds = mfma_lds<R>(a.C, S, !post, a.tiles_per_chunk);
@@ -1053,7 +1227,7 @@ int launch_mfma(bool post, const PassArgs<R>& a, const int32_t* nodes, int width
   dim3 grid(a.nchunks, a.K, width);
 #define HX_MF(KER, MTV, KVV, TV, RV)                                                         \
   do {                                                                                       \
-    auto kern = KER<R, MTV, KVV, TV, RV>;                                                    \
+    auto kern = KER<R, MTV, KVV, TV, RV, SPLIT>;                                             \
     if (lds > 64 * 1024)                                                                     \
       HX_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),                    \
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
@@ -1079,9 +1253,11 @@ int launch_mfma(bool post, const PassArgs<R>& a, const int32_t* nodes, int width
     // MT 4: C <= 4, so C x 16 x 64 products take at most 16 KiB (32 KiB in FP64)
 #define HX_MFR(MTV, KVV)                                                                     \
   if (tips == HX_TIPS_CODES)                                                                 \
-    k_post_mfma_r<R, MTV, KVV, HX_TIPS_CODES><<<grid, threads, lds_r, st>>>(a, nodes, S);    \
+    k_post_mfma_r<R, MTV, KVV, HX_TIPS_CODES, SPLIT><<<grid, threads, lds_r, st>>>(a, nodes, \
+                                                                                   S);       \
   else                                                                                       \
-    k_post_mfma_r<R, MTV, KVV, HX_TIPS_DENSE><<<grid, threads, lds_r, st>>>(a, nodes, S);
+    k_post_mfma_r<R, MTV, KVV, HX_TIPS_DENSE, SPLIT><<<grid, threads, lds_r, st>>>(a, nodes, \
+                                                                                   S);
     if (MT == 1) {
       HX_MFR(1, 4)
     } else if (MT == 4) {
@@ -1117,6 +1293,15 @@ int launch_mfma(bool post, const PassArgs<R>& a, const int32_t* nodes, int width
   return HX_OK;
 }
 
+template <typename R>
+int launch_mfma(bool post, const PassArgs<R>& a, const int32_t* nodes, int width, int tips,
+                bool root, int S, hipStream_t st) {
+  if constexpr (std::is_same<R, float>::value) {
+    if (a.split) return launch_mfma_t<float, true>(post, a, nodes, width, tips, root, S, st);
+  }
+  return launch_mfma_t<R, false>(post, a, nodes, width, tips, root, S, st);
+}
+
 template int launch_mfma<float>(bool, const PassArgs<float>&, const int32_t*, int, int, bool, int,
                                 hipStream_t);
 template int launch_mfma<double>(bool, const PassArgs<double>&, const int32_t*, int, int, bool, int,
@@ -1134,6 +1319,61 @@ __global__ void k_mfma_layout(R* out) {
   vec4_t<R> d = mma(av, bv, vec4_t<R>{R(0), R(0), R(0), R(0)});
 #pragma unroll
   for (int r = 0; r < 4; ++r) out[acc_row<R>(lane >> 4, r) * 16 + (lane & 15)] = d[r];
+}
+
+// The same check for the two BF16 instructions of the split products: A[i][k] = (i + k) % 7 - 3
+// and B[k][j] = (2 k + j) % 5 - 2 (exact in BF16, exact sums in FP32) with k = N g + e in
+// lane i + 16 g (A) or j + 16 g (B), element e; and one full split product (mma3) of
+// non-representable values against its FP64 value.
+template <int N>
+__global__ void k_mfma_layout_bf16(float* out) {
+  const int lane = threadIdx.x;
+  const int i = lane & 15, g = lane >> 4;
+  BfPack<N> a, b, a3, b3;
+#pragma unroll
+  for (int e = 0; e < N; ++e) {
+    const int k = N * g + e;
+    a.hi[e] = (__bf16)(float)((i + k) % 7 - 3);
+    b.hi[e] = (__bf16)(float)((2 * k + i) % 5 - 2);
+    pack_set(a3, e, 1.0f + 0.01f * (float)i + 0.003f * (float)k);
+    pack_set(b3, e, 0.7f - 0.002f * (float)k + 0.013f * (float)i);
+  }
+  vec4_t<float> d = {0.f, 0.f, 0.f, 0.f}, d3 = {0.f, 0.f, 0.f, 0.f};
+  if constexpr (N == 8) d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.hi, d, 0, 0, 0);
+  else d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a.hi, b.hi, d, 0, 0, 0);
+  d3 = mma3(a3, b3, d3);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    out[acc_row<float>(g, r) * 16 + i] = d[r];
+    out[256 + acc_row<float>(g, r) * 16 + i] = d3[r];
+  }
+}
+
+template <int N>
+static int mfma_layout_check_bf16(hipStream_t st, int* bad) {
+  float* d = nullptr;
+  HX_HIP_TRY(hipMalloc(&d, 512 * sizeof(float)));
+  k_mfma_layout_bf16<N><<<1, 64, 0, st>>>(d);
+  float host[512];
+  hipError_t e = hipMemcpyAsync(host, d, sizeof(host), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  hipFree(d);
+  if (e != hipSuccess) {
+    set_error(std::string("mfma bf16 self-test: ") + hipGetErrorString(e));
+    return HX_ERR_HIP;
+  }
+  for (int i = 0; i < 16; ++i)
+    for (int j = 0; j < 16; ++j) {
+      double ref = 0.0, ref3 = 0.0;
+      for (int k = 0; k < 4 * N; ++k) {
+        ref += (double)((i + k) % 7 - 3) * (double)((2 * k + j) % 5 - 2);
+        ref3 += (1.0 + 0.01 * i + 0.003 * k) * (0.7 - 0.002 * k + 0.013 * j);
+      }
+      if ((double)host[i * 16 + j] != ref) ++*bad;
+      // a single BF16 product would be off by ~4e-3; the split form by ~1e-5
+      if (fabs((double)host[256 + i * 16 + j] - ref3) > 1e-4 * fabs(ref3)) ++*bad;
+    }
+  return HX_OK;
 }
 
 template <typename R>
@@ -1166,6 +1406,8 @@ extern "C" int hx_selftest_mfma_layout(int* mismatches, hx_stream_t stream) {
   int bad = 0;
   int rc = hx::mfma_layout_check<float>(st, &bad);
   if (rc == HX_OK) rc = hx::mfma_layout_check<double>(st, &bad);
+  if (rc == HX_OK) rc = hx::mfma_layout_check_bf16<8>(st, &bad);
+  if (rc == HX_OK) rc = hx::mfma_layout_check_bf16<4>(st, &bad);
   *mismatches = bad;
   return rc;
 }

@@ -76,10 +76,12 @@ class TreeJob:
     ``tips``: list of T tensors, [S, N] partials or [N] integer codes in 0..S
     (``tip_kind`` "partials" or "states"), or a ready ``TipData``.
     ``dtype``: torch.float32 (rescaled FP32 partials) or torch.float64.
+    ``products``: for FP32, "fp32" or "bf16x3" (dense-state products as split BF16 on the
+    matrix cores, FP32 accumulation).
     """
 
     def __init__(self, post_indexing, tips, weights, S, dtype=torch.float32, device=None,
-                 tip_kind="partials"):
+                 tip_kind="partials", products="fp32"):
         if device is None:
             device = torch.device("cuda", torch.cuda.current_device())
         device = torch.device(device)
@@ -100,7 +102,9 @@ class TreeJob:
                 td = TipData.from_partials(tips, device)
         self.tips = td
         self.N = td.N
-        self.pruning = Pruning(self.plan, td, torch.as_tensor(weights).to(device))
+        self.products = products
+        self.pruning = Pruning(self.plan, td, torch.as_tensor(weights).to(device),
+                               products=products)
 
     def loglik(self, bls, rates, props, q_norm, freqs):
         """loglik [K] (FP64) with autograd w.r.t. every tensor argument.

@@ -21,6 +21,12 @@ Differences a caller can observe:
   ``props`` through a hand-written pre-order adjoint; the post-order
   workspace is consumed by the backward, so a graph can be backpropagated
   once (``retain_graph`` second passes raise);
+* the tip-states variants read a tip of unknown state (code S) as P times a
+  row of ones, which equals the reference's appended column of ones
+  (``tree_likelihood.py:106-112``) because the rows of P sum to 1; d loglik /
+  d mats therefore carries, for such tips, a component along each row's
+  all-ones direction that the reference's does not. Every derivative through
+  P(t) (branch lengths, rates, Q) is blind to that direction and agrees;
 * CPU inputs are copied to the current HIP device and the result is
   returned on the input device. Without a HIP device these functions raise.
 """
@@ -183,26 +189,48 @@ def get_tips(tips, kind: str, S: int, device: int) -> TipData:
 # ---------------------------------------------------------- core function
 
 
-def _dtype_code(dtype):
+PRODUCTS = ("fp32", "bf16x3")
+_dense_products = "fp32"
+
+
+def set_dense_products(mode: str) -> str:
+    """How FP32 calls of the drop-in functions form their dense-state products (S != 4, at
+    most 4 categories): "fp32" on the FP32 matrix cores, or "bf16x3", split BF16 (hi + lo
+    halves of each operand, three products, FP32 accumulation). Returns the previous mode."""
+    global _dense_products
+    if mode not in PRODUCTS:
+        raise HipTreeError(f"products must be one of {PRODUCTS}")
+    prev, _dense_products = _dense_products, mode
+    return prev
+
+
+def _dtype_code(dtype, products=None):
     if dtype == torch.float64:
         return HX_F64
     if dtype == torch.float32:
-        return HX_F32
+        mode = _dense_products if products is None else products
+        return _lib.HX_F32_BF16X3 if mode == "bf16x3" else HX_F32
     raise HipTreeError(f"unsupported dtype {dtype}")
 
 
 class Pruning:
-    """Bound inputs of one pruning call on the device (no autograd)."""
+    """Bound inputs of one pruning call on the device (no autograd).
 
-    def __init__(self, plan: Plan, tips: TipData, weights: torch.Tensor):
+    ``products``: "fp32" or "bf16x3" for FP32 calls (dense states), or None to follow
+    ``set_dense_products``."""
+
+    def __init__(self, plan: Plan, tips: TipData, weights: torch.Tensor, products=None):
         self.plan = plan
         self.tips = tips
         self.weights = weights.to(torch.float64).contiguous()
         self.S, self.N, self.T = tips.S, tips.N, tips.T
+        if products is not None and products not in PRODUCTS:
+            raise HipTreeError(f"products must be one of {PRODUCTS}")
+        self.products = products
 
     def _common(self, mats, freqs, props, dtype):
         K, B, C, S, _ = mats.shape
-        code = _dtype_code(dtype)
+        code = _dtype_code(dtype, self.products)
         if self.tips.kind == HX_TIPS_CODES:
             tips_p, table_p = self.tips.tips, self.tips.table_as(dtype)
         else:
@@ -214,7 +242,7 @@ class Pruning:
         if S == 4 and which == "post":
             return 2 + int(self.tips.kind == _lib.HX_TIPS_CODES)  # pad codes, walk, root sums
         groups = ((self.N + 63) // 64 + 3) // 4 * K
-        if S == 4 and which == "pre" and code == _lib.HX_F32 and C <= 4 and groups >= 256:
+        if S == 4 and which == "pre" and code != _lib.HX_F64 and C <= 4 and groups >= 256:
             return 2  # walk, statistic reduction (mirrors use_pre_walk, csrc/common.hpp)
         return (self.plan.post_levels if which == "post" else self.plan.pre_levels) + 1
 

@@ -85,10 +85,69 @@ def general_nonsymmetric_q(rates, mapping, pi):
     return q - torch.diag_embed(q.sum(-1))
 
 
-def p_t_expm(q_norm, tau):
-    """P = expm(Q tau) for a normalised non-reversible Q [..., S, S] and tau [..., B, C]
-    (abstract.py:88-94), in the dtype of Q; differentiable through torch."""
-    return torch.matrix_exp(q_norm[..., None, None, :, :] * tau[..., None, None])
+class _ExpmFn(torch.autograd.Function):
+    """mats [K, B, C, S, S] = exp(Q_e bls[k, b] rates[k, c]) on the device (hx_pmatrix_expm),
+    with the backward to Q, bls and rates through the Frechet derivative (hx_pmatrix_expm_grad)."""
+
+    @staticmethod
+    def forward(ctx, q, bls, rates, dtype):
+        lib = _lib.load()
+        K, B = bls.shape
+        E, S, _ = q.shape
+        C = rates.shape[-1]
+        dev = bls.device
+        qd = q.detach().to(dev, F64).contiguous()
+        bd = bls.detach().to(F64).contiguous()
+        rd = rates.detach().to(dev, F64).contiguous()
+        out = torch.empty(K, B, C, S, S, dtype=dtype, device=dev)
+        code = HX_F64 if dtype == F64 else HX_F32
+        with timing.phase("pmatrix_expm"):
+            rc = lib.hx_pmatrix_expm(S, K, B, C, int(E == K and K > 1), _lib.ptr(qd), _lib.ptr(bd),
+                                     _lib.ptr(rd), int(rd.shape[0] == K and K > 1), _lib.ptr(out),
+                                     code, _lib.stream_handle(dev))
+        _lib.check(rc, "hx_pmatrix_expm")
+        ctx.save_for_backward(qd, bd, rd)
+        ctx.shapes = (q.shape, q.device, q.dtype, rates.shape, rates.device, rates.dtype)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        lib = _lib.load()
+        qd, bd, rd = ctx.saved_tensors
+        K, B = bd.shape
+        E, S, _ = qd.shape
+        C = rd.shape[-1]
+        dev = bd.device
+        gd = g.to(F64).contiguous()
+        scratch = torch.empty(int(lib.hx_pmatrix_expm_grad_scratch_bytes(S, K, B, C)),
+                              dtype=torch.uint8, device=dev)
+        dbls = torch.empty(K, B, dtype=F64, device=dev)
+        drates = torch.empty(K, C, dtype=F64, device=dev)
+        dq = torch.empty(E, S, S, dtype=F64, device=dev)
+        rb = int(rd.shape[0] == K and K > 1)
+        with timing.phase("pmatrix_expm_grad"):
+            rc = lib.hx_pmatrix_expm_grad(S, K, B, C, int(E == K and K > 1), _lib.ptr(qd),
+                                          _lib.ptr(bd), _lib.ptr(rd), rb, _lib.ptr(gd),
+                                          _lib.ptr(dbls), _lib.ptr(drates), _lib.ptr(dq),
+                                          _lib.ptr(scratch), _lib.stream_handle(dev))
+        _lib.check(rc, "hx_pmatrix_expm_grad")
+        qshape, qdev, qdt, rshape, rdev, rdt = ctx.shapes
+        if not rb:
+            drates = drates.sum(0, keepdim=True)
+        return (dq.reshape(qshape).to(qdev, qdt), dbls.to(bd.dtype),
+                drates.reshape(rshape).to(rdev, rdt), None)
+
+
+def pmatrix_expm(q_norm, bls, rates, dtype):
+    """P = exp(Q tau) of a normalised rate matrix of any symmetry, tau = bls[k, b] rates[., c]
+    (NonSymmetricSubstitutionModel.p_t, abstract.py:88-94), on the device.
+
+    ``q_norm`` [S, S] or [E, S, S] (E = 1 or K), ``bls`` [K, B] (device), ``rates`` [K|1, C];
+    returns mats [K, B, C, S, S] in ``dtype``, differentiable with respect to all three."""
+    q = q_norm if q_norm.dim() == 3 else q_norm.unsqueeze(0)
+    if rates.dim() == 1:
+        rates = rates.unsqueeze(0)
+    return _ExpmFn.apply(q, bls, rates, dtype)
 
 
 UNIVERSAL_CODE = "KNKNTTTTRSRSIIMIQHQHPPPPRRRRLLLLEDEDAAAAGGGGVVVV*Y*YSSSS*CWCLFLF"

@@ -16,7 +16,7 @@ imports ``torchtree_amd.cli.evolution`` (``cli/evolution.py:700-710,1739-1754``)
 from __future__ import annotations
 
 ENGINE_TYPE = "torchtree_amd.treelikelihood_model.TreeLikelihoodModel"
-PRECISIONS = ("float32", "float64")
+PRECISIONS = ("float32", "float64", "bf16x3")
 
 try:  # torchtree's abstract base when torchtree is importable; the interface otherwise
     from torchtree.cli.plugin import Plugin as _Base  # type: ignore

@@ -7,7 +7,8 @@ every parameter tensor (``requires_grad``):
 * ``TreeLikelihoodModel`` (``evolution/tree_likelihood.py:281-450``): keys ``id``,
   ``tree_model``, ``site_model``, ``substitution_model``, ``site_pattern``, optional
   ``branch_model``, ``use_ambiguities``, ``use_tip_states``; an extra optional
-  ``precision`` key ("float32" | "float64") picks the partials precision
+  ``precision`` key ("float32" | "float64" | "bf16x3": FP32 partials with the dense-state
+  products as split BF16 on the matrix cores) picks the partials precision
   (default: torch's default dtype, as the reference follows it).
 * ``Parameter`` (``core/parameter.py:189``): ``tensor``, ``full``, ``zeros``, ``ones``
   (+ ``value``, ``dtype``, ``requires_grad``).
@@ -703,6 +704,10 @@ class TreeLikelihoodModel:
         self.device = torch.device(device)
         if precision is None:
             precision = torch.get_default_dtype()
+        # "bf16x3": FP32 partials with the dense-state products as split BF16 (hi + lo)
+        self.products = "bf16x3" if precision == "bf16x3" else "fp32"
+        if precision == "bf16x3":
+            precision = torch.float32
         self.dtype = _DTYPES[precision] if isinstance(precision, str) else precision
         topo = tree_model.topo
         if isinstance(site_pattern, AttributePattern):
@@ -730,7 +735,7 @@ class TreeLikelihoodModel:
     def _finish(self, topo, tips, weights, S):
         self.weights = weights
         self.job = TreeJob([tuple(int(x) for x in r) for r in topo.postorder], tips, weights, S,
-                           dtype=self.dtype, device=self.device)
+                           dtype=self.dtype, device=self.device, products=self.products)
 
     @property
     def sample_shape(self):
@@ -768,15 +773,16 @@ class TreeLikelihoodModel:
         return out.reshape(shape + (1,))
 
     def _loglik_expm(self, bls, rates, probs, q, f):
-        """Non-reversible Q: P = expm(Q bl r) by torch.matrix_exp in FP64 on the device
-        (abstract.py:88-94), then the pruning kernels and their d/dP through the drop-in
-        autograd function; the gradient reaches Q through matrix_exp's backward."""
+        """Non-reversible Q: P = exp(Q bl r) by the HIP matrix exponential in FP64
+        (hx_pmatrix_expm; abstract.py:88-94 uses torch.matrix_exp), then the pruning kernels
+        and their d/dP through the drop-in autograd function; the gradient reaches Q, the
+        branch lengths and the rates through the Frechet derivative (hx_pmatrix_expm_grad)."""
         dev = self.device
         q = q.to(dev, torch.float64)
         if q.dim() == 2:
             q = q.unsqueeze(0)
-        tau = bls.to(dev, torch.float64)[:, :, None] * rates.to(dev, torch.float64)[:, None, :]
-        mats = hm.p_t_expm(q, tau).to(self.dtype).contiguous()  # [K, B, C, S, S]
+        mats = hm.pmatrix_expm(q, bls.to(dev, torch.float64), rates.to(dev, torch.float64),
+                               self.dtype)  # [K, B, C, S, S]
         f2 = f.to(dev, self.dtype).reshape(-1, f.shape[-1]).contiguous()
         p2 = probs.to(dev, self.dtype).contiguous()
         return _PruneFn.apply(mats, f2, p2, self.job.pruning)
