@@ -252,6 +252,21 @@ int hx_job_eval_model(hx_job_t job, int K, const double* bls, int eigen_batched,
                       double* out_dq, double* out_dfreqs, double* out_dprops, void* workspace,
                       size_t workspace_bytes, hx_stream_t stream);
 
+/* hx_job_eval without a caller workspace and without any device allocation.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). The workspace is a library-owned static device array of
+ * HX_NOALLOC_WORKSPACE_BYTES per device, held until the process ends; a
+ * request whose hx_job_workspace_bytes(job, K, out_dbls != NULL) exceeds it
+ * returns HX_ERR_WORKSPACE_TOO_SMALL before anything is launched (register the
+ * job with a smaller pattern slice to fit). Every call on one device shares
+ * that device's array, which keeps the last call's contents: calls that
+ * overlap on one GPU (other streams, other host threads) race. Results are
+ * those of hx_job_eval on the same inputs. */
+#define HX_NOALLOC_WORKSPACE_BYTES (64u << 20)
+int hx_job_eval_noalloc(hx_job_t job, int K, const double* bls, double* out_loglik,
+                        double* out_dbls, hx_stream_t stream);
+
 /* Checks on the device the operand/accumulator lane layout the MFMA
  * pruning kernels assume for v_mfma_f32_16x16x4f32; *mismatches = number of
  * wrong output elements (0 expected). No reference counterpart. */

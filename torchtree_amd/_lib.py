@@ -71,11 +71,14 @@ _SIGNATURES = {
     "hx_job_workspace_bytes": (_c_size, [_vp, _c_int, _c_int]),
     "hx_job_workspace_bytes_for": (_c_size, [_c_int] * 7),
     "hx_job_eval": (_c_int, [_vp, _c_int, _dp, _dp, _dp, _vp, _c_size, _vp]),
+    "hx_job_eval_noalloc": (_c_int, [_vp, _c_int, _dp, _dp, _dp, _vp]),
     "hx_job_model_workspace_bytes": (_c_size, [_vp, _c_int, _c_int]),
     "hx_job_eval_model": (_c_int, [_vp, _c_int, _dp, _c_int, _dp, _dp, _dp, _dp, _dp, _dp, _c_int,
                                    _dp, _dp, _dp, _dp, _dp, _dp, _vp, _c_size, _vp]),
     "hx_selftest_mfma_layout": (_c_int, [ctypes.POINTER(_c_int), _vp]),
 }
+
+HX_NOALLOC_WORKSPACE_BYTES = 64 << 20  # include/hiptree.h
 
 HX_GRAD_NONE = 0
 HX_GRAD_MATS = 1

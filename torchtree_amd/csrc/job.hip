@@ -13,6 +13,7 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -293,7 +294,7 @@ size_t hx_job_workspace_bytes_for(int taxa, int S, int slice_patterns, int C, in
 
 namespace hx {
 
-// One request of hx_job_eval in a workspace laid out by job_ws.
+// One request of hx_job_eval / hx_job_eval_noalloc in a workspace laid out by job_ws.
 static int job_eval_in(hx_job_t job, int K, const double* bls, double* out_loglik,
                        double* out_dbls, const JobWs& w, void* workspace, hx_stream_t stream) {
   const int grad = out_dbls != nullptr;
@@ -407,6 +408,37 @@ static int job_eval_model_in(hx_job_t job, int K, const double* bls, const Model
   return HX_OK;
 }
 
+// Workspace of hx_job_eval_noalloc: a file-scope device array, one copy per device, held from
+// the library's load to the end of the process. In host code `scratch` names a host-side
+// stand-in, not a device address: kernels and copies get the per-device address that
+// hipGetSymbolAddress returns (looked up once per device, with that device current).
+__device__ __align__(256) unsigned char scratch[HX_NOALLOC_WORKSPACE_BYTES];
+
+constexpr int kScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_addr[kScratchDevices] = {};
+
+static int scratch_address(int device, void** out) {
+  HX_REQUIRE(device >= 0 && device < kScratchDevices, HX_ERR_INVALID_ARGUMENT,
+             "device index out of range for the static workspace");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_addr[device] == nullptr) {
+    int prev = 0;
+    HX_HIP_TRY(hipGetDevice(&prev));
+    if (prev != device) HX_HIP_TRY(hipSetDevice(device));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (prev != device) hipSetDevice(prev);
+    if (e != hipSuccess || p == nullptr) {
+      set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      return HX_ERR_HIP;
+    }
+    g_scratch_addr[device] = p;
+  }
+  *out = g_scratch_addr[device];
+  return HX_OK;
+}
+
 }  // namespace hx
 
 extern "C" {
@@ -439,6 +471,23 @@ int hx_job_eval_model(hx_job_t job, int K, const double* bls, int eigen_batched,
   const hx::ModelArgs a{eigen_batched, lambda, left, right, rates, props, freqs, model_batched,
                         out_loglik, out_dbls, out_drates, out_dq, out_dfreqs, out_dprops};
   return hx::job_eval_model_in(job, K, bls, a, w, workspace, stream);
+}
+
+int hx_job_eval_noalloc(hx_job_t job, int K, const double* bls, double* out_loglik,
+                        double* out_dbls, hx_stream_t stream) {
+  HX_REQUIRE(job && bls && out_loglik && K >= 1, HX_ERR_INVALID_ARGUMENT, "null argument");
+  const hx::JobWs w = hx::job_ws(job, K, out_dbls != nullptr);
+  if (w.total > (size_t)HX_NOALLOC_WORKSPACE_BYTES) {
+    hx::set_error("request needs " + std::to_string(w.total) +
+                  " bytes of workspace; the static workspace holds " +
+                  std::to_string((size_t)HX_NOALLOC_WORKSPACE_BYTES) +
+                  " (register the job with a smaller pattern slice, or use hx_job_eval)");
+    return HX_ERR_WORKSPACE_TOO_SMALL;
+  }
+  void* ws = nullptr;
+  const int rc = hx::scratch_address(job->device, &ws);
+  if (rc != HX_OK) return rc;
+  return hx::job_eval_in(job, K, bls, out_loglik, out_dbls, w, ws, stream);
 }
 
 }  // extern "C"
