@@ -52,9 +52,10 @@ def parse():
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--config", default="c2", choices=("c1", "c2", "c3", "c4a", "c4b"),
                     help="workload of the timed step (c2 is the headline configuration)")
-    ap.add_argument("--products", default="fp32", choices=("fp32", "bf16x3"),
+    ap.add_argument("--products", default="auto", choices=("auto", "fp32", "bf16x3"),
                     help="dense-state products of the FP32 step: FP32 matrix cores, or split "
-                         "BF16 (hi + lo halves, three products, FP32 accumulation)")
+                         "BF16 (hi + lo halves, three products, FP32 accumulation); auto: split "
+                         "BF16 for the 20- and 61-state workloads, which specify it")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-mix measurement")
     ap.add_argument("--serve-jobs", type=int, default=200)
     ap.add_argument("--serve-requests", type=int, default=200)
@@ -67,7 +68,10 @@ def parse():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run one device-to-device copy of BYTES (a known-size kernel "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters)")
-    return ap.parse_args()
+    args = ap.parse_args()
+    if args.products == "auto":
+        args.products = "bf16x3" if args.config in ("c4a", "c4b") else "fp32"
+    return args
 
 
 def log(msg):

@@ -195,3 +195,95 @@ def test_served_model_request_matches_oracle(gpu, dtype, rl, rg, batched):
         a2 = a[:, :-1] if name == "bls" else a
         err = float((a2 - b2).norm() / b2.norm())
         assert err < rg, (name, err)
+
+
+def test_graph_replays_match_eager(gpu):
+    """Small requests served by replaying captured HIP graphs (one per job, K and gradient
+    flag): every replay gives the eager result bit for bit. The server runs three times over
+    the same graphs, with eager requests of larger jobs in the mix and the program's own eager
+    work and host synchronisations between the runs."""
+    print("case: serving graph replays == eager", flush=True)
+    from torchtree_amd import serving, synthetic
+
+    specs, jobs = _jobs(gpu, torch.float32)
+    K_of = [1, 3, 2, 1, 3]
+    sizes = sorted(jb.workspace_bytes(K_of[i], True) for i, jb in enumerate(jobs))
+    limit = sizes[2]  # the three smallest are graphed, the other two run eagerly
+    srv = serving.Server(32 << 30, n_streams=4, pool_bytes=16 << 30, device=gpu.index or 0,
+                         graph_max_bytes=limit)
+    for jb in jobs:
+        srv.register(jb.name, jb)
+    x = torch.randn(512, 512, device=gpu)
+    graphed = 0
+    for rep in range(3):
+        reqs = [serving.Request(i, jobs[j], synthetic.draw_bls(specs[j], K_of[j], 100 * rep + i),
+                                arrival=0.0005 * i)
+                for i, j in enumerate([0, 1, 2, 3, 4] * 4)]
+        stats = srv.run(reqs)
+        assert stats["requests"] == len(reqs)
+        graphed += stats["graphed"]
+        ref = [serving.Request(r.rid, r.job, r.bls) for r in reqs]
+        serving.run_sequential(ref, device=gpu.index or 0)
+        for a, b in zip(reqs, ref):
+            assert torch.equal(a.loglik, b.loglik), (rep, a.rid, a.graphed)
+            assert torch.equal(a.dbls, b.dbls), (rep, a.rid, a.graphed)
+        # eager work of the program itself between the runs
+        x = torch.tanh(x @ x.t() / 512)
+        torch.cuda.synchronize(gpu)
+    assert graphed >= 6, graphed
+    assert len(srv.executor.graphs) >= 3
+    # unregistering a job drops its graphs and gives their workspace back
+    n_before = len(srv.executor.graphs)
+    in_use = srv.pool.in_use
+    small = min(range(len(jobs)), key=lambda i: jobs[i].workspace_bytes(K_of[i], True))
+    srv.unregister(jobs[small].name)
+    assert len(srv.executor.graphs) < n_before and srv.pool.in_use < in_use
+
+
+def test_server_model_requests(gpu):
+    """Requests that carry their own GTR model (exchangeabilities, frequencies, Weibull rates)
+    go through the scheduler with plain requests in the mix; each equals the same request run
+    alone (hx_job_eval_model), bit for bit, including d/dQ, d/dfreqs, d/drates and d/dprops."""
+    print("case: server with per-request models", flush=True)
+    from torchtree_amd import models as hm
+    from torchtree_amd import serving, synthetic
+
+    specs, jobs = _jobs(gpu, torch.float32)
+    dev = torch.device("cuda", gpu.index or 0)
+    g = torch.Generator().manual_seed(5)
+
+    def model(K, batched, C):
+        lead = (K,) if batched else (1,)
+        ex = torch.rand(lead + (6,), generator=g, dtype=F64) + 0.2
+        fr = torch.rand(lead + (4,), generator=g, dtype=F64) + 0.5
+        fr = fr / fr.sum(-1, keepdim=True)
+        qn = hm.normalize(hm.gtr_q(ex, fr), fr)
+        rates, props = hm.weibull_rates(torch.tensor([0.7], dtype=F64), C)
+        R = lead[0]
+        eig = hm.Eigen(qn if batched else qn[0], fr if batched else fr[0], dev)
+        return {"eig": eig, "rates": rates.reshape(1, C).expand(R, C).contiguous().to(dev),
+                "props": props.reshape(1, C).expand(R, C).contiguous().to(dev),
+                "freqs": fr.to(dev).contiguous()}
+
+    dna = [i for i, sp in enumerate(specs) if sp["kind"] == "dna"]
+    reqs = []
+    for i in range(16):
+        j = dna[i % len(dna)] if i % 2 == 0 else i % len(jobs)
+        K = 1 + i % 3
+        m = model(K, (i % 4 == 0), jobs[j].C) if i % 2 == 0 else None
+        reqs.append(serving.Request(i, jobs[j], synthetic.draw_bls(specs[j], K, i),
+                                    arrival=0.0003 * i, model=m))
+    srv = serving.Server(32 << 30, n_streams=4, pool_bytes=16 << 30, device=gpu.index or 0)
+    for jb in jobs:
+        srv.register(jb.name, jb)
+    stats = srv.run(reqs)
+    assert stats["requests"] == len(reqs)
+    ref = [serving.Request(r.rid, r.job, r.bls, model=r.model) for r in reqs]
+    serving.run_sequential(ref, device=gpu.index or 0)
+    for a, b in zip(reqs, ref):
+        assert torch.isfinite(a.loglik).all()
+        assert torch.equal(a.loglik, b.loglik), a.rid
+        assert torch.equal(a.dbls, b.dbls), a.rid
+        if a.model is not None:
+            for key in ("drates", "dq", "dfreqs", "dprops"):
+                assert torch.equal(a.grads[key], b.grads[key]), (a.rid, key)

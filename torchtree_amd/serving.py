@@ -224,10 +224,34 @@ class Request:
     cancelled: bool = False    # set by Server.cancel; a queued request is then dropped
     bypassed: int = 0          # later requests admitted while this one waited for memory
     graphed: bool = False      # served by replaying a captured graph
+    # A request's own model (hx_job_eval_model) instead of the job's registered one:
+    # {"eig": models.Eigen on the job's device, "rates", "props" [R, C], "freqs" [R, S]}
+    # device FP64 with R = 1 or K. Its results then also fill ``grads`` (drates, dq, dfreqs,
+    # dprops, as ResidentJob.eval_model returns them).
+    model: dict | None = None
+    grads: dict | None = None
 
     @property
     def K(self):
         return self.bls.shape[0]
+
+    def workspace_bytes(self):
+        if self.model is not None:
+            return self.job.model_workspace_bytes(self.K, self.model["eig"].batched)
+        return self.job.workspace_bytes(self.K, self.grad)
+
+    def run_on(self, bls_dev, ws, stream):
+        """Evaluate on ``stream`` into fresh device outputs (``loglik``, ``dbls``, ``grads``)."""
+        dev = bls_dev.device
+        if self.model is not None:
+            m = self.model
+            out = self.job.eval_model(bls_dev, m["eig"], m["rates"], m["props"], m["freqs"], ws,
+                                      stream.cuda_stream)
+            self.loglik, self.dbls, self.grads = out["loglik"], out["dbls"], out
+            return
+        self.loglik = torch.empty(self.K, dtype=F64, device=dev)
+        self.dbls = torch.empty(self.K, self.job.B, dtype=F64, device=dev) if self.grad else None
+        self.job.eval(bls_dev, self.loglik, self.dbls, ws, stream.cuda_stream)
 
     @property
     def latency(self):
@@ -325,9 +349,9 @@ class GpuExecutor:
     def graph_for(self, req):
         """The captured graph that will serve ``req`` (captured now on first sight of its job,
         K and gradient flag), or None when the request runs eagerly."""
-        if not self.graph_max_bytes:
+        if not self.graph_max_bytes or req.model is not None:
             return None
-        need = req.job.workspace_bytes(req.K, req.grad)
+        need = req.workspace_bytes()
         if need > self.graph_max_bytes:
             return None
         key = (id(req.job), req.K, req.grad)
@@ -366,7 +390,6 @@ class GpuExecutor:
 
     def start(self, slot, req: Request, ws_view, ent=None):
         st = self.streams[slot]
-        job = req.job
         with torch.cuda.stream(st):
             bls = req.bls.to(self.device, non_blocking=True)
             if ent is not None:
@@ -377,12 +400,18 @@ class GpuExecutor:
                 req.dbls = ent["db"].clone() if req.grad else None
                 req.graphed = True
             else:
-                req.loglik = torch.empty(req.K, dtype=F64, device=self.device)
-                req.dbls = torch.empty(req.K, job.B, dtype=F64, device=self.device) if req.grad else None
-                job.eval(bls, req.loglik, req.dbls, ws_view, st.cuda_stream)
+                req.run_on(bls, ws_view, st)
             ev = torch.cuda.Event()
             ev.record(st)
         self.events[slot] = (ev, bls, ent)
+
+    def drop_job(self, job):
+        """Forget the graphs captured for ``job``; their buffers go back to the pool."""
+        for key in [k for k, v in self.graphs.items() if v["job"] is job]:
+            ent = self.graphs.pop(key)
+            if ent["busy"]:
+                torch.cuda.synchronize(self.device)
+            self.pool.release(ent["ws_off"], ent["ws_n"])
 
     def done(self, slot):
         ev = self.events[slot]
@@ -420,6 +449,9 @@ class FakeExecutor:
 
     def graph_for(self, req):
         return None
+
+    def drop_job(self, job):
+        pass
 
     def start(self, slot, req, ws_view, ent=None):
         self.busy_until[slot] = self.clock() + self.cost(req)
@@ -514,7 +546,9 @@ class Server:
         return job
 
     def unregister(self, name):
+        """Remove a job (no request of it may be queued or running)."""
         job = self.jobs.pop(name)
+        self.executor.drop_job(job)
         self.resident -= job.resident_bytes
 
     @staticmethod
@@ -534,7 +568,7 @@ class Server:
                 r.status = "cancelled"
                 self.counters["cancelled"] += 1
                 continue
-            need = r.job.workspace_bytes(r.K, r.grad)
+            need = r.workspace_bytes()
             if need > min(self.max_request_bytes, self.pool.size):
                 queue.remove(r)
                 r.status = "rejected"
@@ -635,7 +669,7 @@ def run_sequential(requests, device=0, ws_bytes=None, time_limit=None, progress=
     (``partial``).
     """
     dev = torch.device("cuda", device)
-    need = max(r.job.workspace_bytes(r.K, r.grad) for r in requests)
+    need = max(r.workspace_bytes() for r in requests)
     ws = torch.empty(int(need if ws_bytes is None else ws_bytes), dtype=torch.uint8, device=dev)
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
@@ -653,10 +687,7 @@ def run_sequential(requests, device=0, ws_bytes=None, time_limit=None, progress=
                 time.sleep(wait)
         done.append(r)
         a = time.perf_counter()
-        bls = r.bls.to(dev)
-        r.loglik = torch.empty(r.K, dtype=F64, device=dev)
-        r.dbls = torch.empty(r.K, r.job.B, dtype=F64, device=dev) if r.grad else None
-        r.job.eval(bls, r.loglik, r.dbls, ws, torch.cuda.current_stream(dev).cuda_stream)
+        r.run_on(r.bls.to(dev), ws, torch.cuda.current_stream(dev))
         torch.cuda.synchronize(dev)
         r.t_start = a - t0
         if not realtime:
