@@ -24,7 +24,7 @@ int main() {
   std::printf("%d %d\n", need, nseg);
   for (int i = 0; i < I; ++i) {
     const int32_t* w = walk.data() + hx::kWalkInts * i;
-    std::printf("%d %d %d %d %d %d %d\n", w[0], w[1], w[2], w[3], w[4], w[5], w[6]);
+    std::printf("%d %d %d %d %d %d %d %d\n", w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7]);
   }
   for (int s = 0; s < nseg; ++s) {
     for (int j = 0; j < hx::kWalkSegTips; ++j)

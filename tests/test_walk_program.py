@@ -51,7 +51,22 @@ def _check(T, post, need, walk):
     seg = -1
     used = []
     prev = None
-    for v, a, b, slots, tpos, sopen, kinds in prog:
+    # drop bits: who consumes each node, and how
+    consumed = {}
+    for v, a, b, slots, tpos, sopen, kinds, drop in prog:
+        for ch, sl, kind in ((a, (slots & 0xFF) - 1, kinds & 0xFF),
+                             (b, ((slots >> 8) & 0xFF) - 1, kinds >> 8)):
+            consumed[ch] = (kind, sl)
+    for v, a, b, slots, tpos, sopen, kinds, drop in prog:
+        cherry = a < T and b < T
+        kind, sl = consumed.get(v, (None, None))
+        in_regs = 2 <= sl < 5 if sl is not None else False
+        # mode 0: only a cherry whose parent takes it as the previous result; mode 1 also
+        # one waiting at a stack depth the post-order kernel keeps in registers (2..4)
+        assert (drop & 1) == (1 if cherry and kind == 1 else 0), (v, drop)
+        assert (drop >> 1) == (1 if cherry and (kind == 1 or (kind == 2 and in_regs)) else 0)
+        assert drop & ~3 == 0
+    for v, a, b, slots, tpos, sopen, kinds, drop in prog:
         assert kids[v] == (a, b) and v not in done
         sa, sb, so = (slots & 0xFF) - 1, ((slots >> 8) & 0xFF) - 1, slots >> 16
         pa, pb = (tpos & 0xFF) - 1, ((tpos >> 8) & 0xFF) - 1
@@ -165,8 +180,22 @@ def _check_pre(T, post, depth, prog):
     kids = {v: (a, b) for v, a, b in post}
     root = post[-1][0]
     assert len(prog) == T - 1 and prog[0][0] == root and prog[0][3] == 0
+    cherries = [v for v, a, b in post if a < T and b < T]  # rows T + i, in post-order
     seen, stack, nxt = set(), [], None
-    for p, a, b, usrc, cont, push, tips in prog:
+    for p, a, b, usrc, cont, push, tips, drop, row_a, row_b, a0, a1, b0, b1 in prog:
+        # a dropped child (either register mode) is a cherry, rebuilt from its two tips,
+        # whose codes are row T + its index among the cherries
+        for h, (ch, row, g0, g1) in enumerate(((a, row_a, a0, a1), (b, row_b, b0, b1))):
+            bits = (drop >> (2 * h)) & 3
+            if ch < T:
+                assert row == ch and bits == 0 and g0 == g1 == 0
+            elif ch in cherries:
+                assert row == T + cherries.index(ch) and (g0, g1) == kids[ch]
+                assert bits in (0, 2, 3)  # mode 1 drops whatever mode 0 drops
+            else:
+                assert row == 0 and bits == 0 and g0 == g1 == 0
+        assert drop & ~15 == 0
+    for p, a, b, usrc, cont, push, tips, *_ in prog:
         assert kids[p] == (a, b) and p not in seen
         seen.add(p)
         if usrc == 1:
@@ -212,10 +241,13 @@ def test_pre_walk_shapes(pre_walk_bin, shape, T):
         assert depth == int(math.log2(T)) - 1
 
 
-def _pre_walk_statistics(T, post, prog, tips, mats, freqs, props, weights, rs):
+def _pre_walk_statistics(T, post, prog, tips, mats, freqs, props, weights, rs, mode=None):
     """The kernel's algorithm restated in FP64 numpy: post-order partials are
     overwritten in place by adjoints pushed at depth >= rs, shallower pushes
-    live in `rs` register slots; returns d loglik / d mats [B, C, S, S]."""
+    live in `rs` register slots; returns d loglik / d mats [B, C, S, S].
+    ``mode`` (0 or 1): the cherries whose drop bit of that register mode is set
+    were never stored, and the walk rebuilds them from the two tips its entry
+    names (reading such a slot fails)."""
     C, S = mats.shape[1], mats.shape[2]
     post_l = {}
     for t in range(T):
@@ -224,10 +256,15 @@ def _pre_walk_statistics(T, post, prog, tips, mats, freqs, props, weights, rs):
         post_l[v] = np.einsum("cij,cjn->cin", mats[a], post_l[a]) * np.einsum(
             "cij,cjn->cin", mats[b], post_l[b])
     mem = {v: ("L", post_l[v]) for v in range(T, 2 * T - 1)}
+    if mode is not None:
+        for p, a, b, usrc, cont, push, tipf, drop, *_ in prog:
+            for h, x in enumerate((a, b)):
+                if (drop >> (2 * h + mode)) & 1:
+                    mem[x] = ("never stored", None)
     regs = [None] * rs
     grad = np.zeros_like(mats)
     U = None
-    for p, a, b, usrc, cont, push, tipf in prog:
+    for p, a, b, usrc, cont, push, tipf, drop, row_a, row_b, a0, a1, b0, b1 in prog:
         if usrc == 0:
             U = np.broadcast_to(freqs[:, None], (C, S, tips.shape[-1]))
         elif usrc != 1:
@@ -238,9 +275,13 @@ def _pre_walk_statistics(T, post, prog, tips, mats, freqs, props, weights, rs):
                 kind, U = mem[p]
                 assert kind == "U"  # the in-place slot holds the pushed adjoint
         L = []
-        for x in (a, b):
+        for h, (x, g) in enumerate(((a, (a0, a1)), (b, (b0, b1)))):
             if x < T:
                 L.append(post_l[x])
+            elif mode is not None and (drop >> (2 * h + mode)) & 1:
+                # rebuilt from the tips the entry names (nothing was stored)
+                L.append(np.einsum("cij,cjn->cin", mats[g[0]], post_l[g[0]]) *
+                         np.einsum("cij,cjn->cin", mats[g[1]], post_l[g[1]]))
             else:
                 kind, val = mem[x]
                 assert kind == "L"  # read before any push overwrote it
@@ -266,9 +307,11 @@ def _pre_walk_statistics(T, post, prog, tips, mats, freqs, props, weights, rs):
     return grad
 
 
+@pytest.mark.parametrize("mode", [None, 0, 1])
 @pytest.mark.parametrize("T,rs", [(2, 2), (9, 0), (40, 1), (40, 2), (130, 2)])
-def test_pre_walk_statistics_match_oracle(pre_walk_bin, T, rs):
-    """The walk's branch statistics equal the oracle's autograd gradient."""
+def test_pre_walk_statistics_match_oracle(pre_walk_bin, T, rs, mode):
+    """The walk's branch statistics equal the oracle's autograd gradient, also when
+    the cherries a register mode drops are rebuilt from their tips."""
     import torch
 
     from oracle import pruning as orc
@@ -284,7 +327,7 @@ def test_pre_walk_statistics_match_oracle(pre_walk_bin, T, rs):
     props = rng.dirichlet(np.ones(C))
     tips = rng.uniform(0.05, 1.0, size=(T, S, N))
     weights = rng.integers(1, 5, size=N).astype(np.float64)
-    got = _pre_walk_statistics(T, post, prog, tips, mats, freqs, props, weights, rs)
+    got = _pre_walk_statistics(T, post, prog, tips, mats, freqs, props, weights, rs, mode)
 
     tm = torch.tensor(mats, requires_grad=True)
     parts = [torch.tensor(tips[t]) for t in range(T)] + [None] * (T - 1)

@@ -96,14 +96,17 @@ int hx_plan_create(int taxa, const int32_t* postorder, int device, hx_plan_t* ou
     pre_lv[depth[v]].push_back(v);
   }
   std::vector<int32_t> walk, segtab;
+  std::vector<uint8_t> drop;
   int walk_need = 0;
-  if (!hx::build_walk(T, kids, postorder, walk, segtab, walk_need)) {
+  if (!hx::build_walk(T, kids, postorder, walk, segtab, walk_need, &drop)) {
     hx::set_error("walk program construction failed");
     return HX_ERR_BAD_TOPOLOGY;
   }
+  std::vector<int32_t> cher, cher_row;
+  hx::list_cherries(T, kids, postorder, cher, cher_row);
   std::vector<int32_t> prew;
   int prew_depth = 0;
-  if (!hx::build_pre_walk(T, kids, postorder, prew, prew_depth)) {
+  if (!hx::build_pre_walk(T, kids, postorder, prew, prew_depth, &drop, &cher_row)) {
     hx::set_error("pre-order walk program construction failed");
     return HX_ERR_BAD_TOPOLOGY;
   }
@@ -117,10 +120,12 @@ int hx_plan_create(int taxa, const int32_t* postorder, int device, hx_plan_t* ou
   const size_t walk_off = hx::align_up(2 * (size_t)nn + 2 * (size_t)I, 4);
   const size_t seg_off = walk_off + walk.size();
   const size_t prew_off = hx::align_up(seg_off + segtab.size(), 4);
-  std::vector<int32_t> host(prew_off + prew.size(), 0);
+  const size_t cher_off = prew_off + prew.size();
+  std::vector<int32_t> host(cher_off + cher.size(), 0);
   std::copy(walk.begin(), walk.end(), host.begin() + walk_off);
   std::copy(segtab.begin(), segtab.end(), host.begin() + seg_off);
   std::copy(prew.begin(), prew.end(), host.begin() + prew_off);
+  std::copy(cher.begin(), cher.end(), host.begin() + cher_off);
   std::copy(kids.begin(), kids.end(), host.begin());
   int pos = 2 * nn;
   plan->post_off.push_back(0);
@@ -154,6 +159,8 @@ int hx_plan_create(int taxa, const int32_t* postorder, int device, hx_plan_t* ou
   plan->nseg = (int)(segtab.size() / hx::kWalkSegTips);
   plan->d_prew = plan->d_buf + prew_off;
   plan->prew_depth = prew_depth;
+  plan->d_cher = plan->d_buf + cher_off;
+  plan->ncher = (int)(cher.size() / 2);
   *out = plan;
   return HX_OK;
 }

@@ -124,7 +124,8 @@ struct hx_plan_s {
   int I = 0;
   int root = 0;
   int32_t* d_buf = nullptr;   // device: kids[2(2T-1)] | post_nodes[I] | pre_nodes[I] | pad |
-                              //         walk[I][8] (16-byte aligned) | segtab[nseg][16]
+                              //         walk[I][8] (16-byte aligned) | segtab[nseg][16] |
+                              //         prew[I][16] | cher[ncher][2]
   const int32_t* d_kids = nullptr;
   const int32_t* d_post = nullptr;
   const int32_t* d_pre = nullptr;
@@ -141,4 +142,7 @@ struct hx_plan_s {
   // internal node; its stack reaches `prew_depth` entries.
   const int32_t* d_prew = nullptr;
   int prew_depth = 0;
+  // Cherries (walk.hpp, list_cherries): [ncher][2] tip ids.
+  const int32_t* d_cher = nullptr;
+  int ncher = 0;
 };

@@ -133,7 +133,7 @@ __global__ __launch_bounds__(256) void k_pad_mats(const R* __restrict__ mats, R*
 
 // ------------------------------------------------------------------- host side
 struct WsLayout {
-  size_t partials, tail, scales, rootsums, gpart, ppad, total;  // tail: padding after the partials
+  size_t partials, tail, scales, rootsums, gpart, ppad, pcodes, total;  // tail: padding after the partials
 };
 
 // FP32 gradients at 33..64 states read P and P^T from zero-padded 64 x 64 copies.
@@ -166,8 +166,24 @@ static WsLayout ws_layout_t(int taxa, int S, int N, int C, int K, int dtype, int
   if (grad_mode == 2) off = align_up(off + B * K * (size_t)t.nchunks * C * 8, 256);
   w.ppad = off;
   if (use_ppad(S, dtype, grad_mode)) off = align_up(off + B * K * (size_t)C * 2 * 64 * 64 * rb, 256);
+  // four-state FP32: 16-bit tip codes and cherry code pairs for the pre-order walk (a tree
+  // has at most T / 2 cherries)
+  w.pcodes = off;
+  if (S == 4 && rb == 4)
+    off = align_up(off + ((size_t)taxa + (size_t)taxa / 2) * t.ntiles * 64 * 2, 256);
   w.total = off;
   return w;
+}
+
+// Register mode of the post-order walk's drop bits (walk.hpp) for a call, or -1 when every
+// internal partial is stored: cherries are dropped for four-state FP32 calls with tip codes
+// whose gradient runs as the pre-order walk (use_pre_walk), the only reader that rebuilds
+// them. Mode 1 where the post-order walk keeps stack depths in registers (FP32 at 1, 2 or 4
+// categories), else mode 0.
+static int walk_drop_mode(int S, int C, int N, int K, int dtype, int tip_kind) {
+  if (S != 4 || dtype == HX_F64 || tip_kind != HX_TIPS_CODES || !use_pre_walk(S, C, 4, N, K))
+    return -1;
+  return (C == 1 || C == 2 || C == 4) ? 1 : 0;
 }
 
 static WsLayout ws_layout(const hx_plan_s* plan, int S, int N, int C, int K, int dtype,
@@ -264,6 +280,10 @@ static PassArgs<R> make_args(const CallArgs& c) {
   a.gtau = reinterpret_cast<double*>(base + w.gpart);
   a.ppad = use_ppad(c.S, c.dtype, c.grad_mode) ? reinterpret_cast<R*>(base + w.ppad) : nullptr;
   a.split = c.dtype == HX_F32_BF16X3;
+  a.drop_mode = walk_drop_mode(c.S, c.C, c.N, c.K, c.dtype, c.tip_kind);
+  a.cher = c.plan->d_cher;
+  a.ncher = c.plan->ncher;
+  a.pcodes = c.S == 4 && sizeof(R) == 4 ? reinterpret_cast<uint16_t*>(base + w.pcodes) : nullptr;
   return a;
 }
 

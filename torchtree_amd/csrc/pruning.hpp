@@ -31,6 +31,13 @@ struct PassArgs {
   R* ppad;                // [K][B][C][2][64][64]: P and P^T zero-padded (FP32, 33..64 states);
                           // with `split`, each element holds its BF16 hi and lo halves
   int split;              // dense-state products as split BF16 on the matrix cores (FP32 only)
+  // Four-state FP32 walks with tip codes: -1, or the register mode whose drop bits
+  // (walk.hpp) apply: the post-order walk stores no partial of those cherries and the
+  // pre-order walk rebuilds it from pcodes.
+  int drop_mode;
+  const int32_t* cher;    // [ncher][2] tip ids of the cherries
+  int ncher;
+  uint16_t* pcodes;       // [T + ncher][np]: tip codes, then each cherry's two codes (lo | hi << 8)
 };
 
 constexpr double kLn2 = 0.69314718055994530942;

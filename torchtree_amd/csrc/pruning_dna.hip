@@ -200,12 +200,25 @@ __device__ __forceinline__ void acc_outer(R (&g)[16], R w, const V4<R>& u, const
 // CM: categories rounded up to a power of two (EXACT: C == CM).
 // NRS > 0: results at stack depths kRegDepth0 .. kRegDepth0 + NRS - 1 also
 // stay in registers, so their consumers do not read them back from memory.
-constexpr int kRegDepth0 = 2;
+// a.drop_mode >= 0: the cherries whose drop bit (walk.hpp) is set are not
+// stored; their store instruction is still issued, as a raw buffer store
+// with an empty range (the hardware discards it), so every entry issues the
+// same vector memory operations and the waits stay exact.
+constexpr int kRegDepth0 = kWalkRegDepth0;
+
+__device__ __forceinline__ void store_row_f32(const __amdgpu_buffer_rsrc_t& r, unsigned off,
+                                              const V4<float>& v) {
+  typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+  const u4 w = {__float_as_uint(v.x[0]), __float_as_uint(v.x[1]), __float_as_uint(v.x[2]),
+                __float_as_uint(v.x[3])};
+  __builtin_amdgcn_raw_buffer_store_b128(w, r, (int)off, 0, 0);
+}
 
 template <typename R, int TIPS, int CM, bool EXACT, int NRS>
 __global__ __launch_bounds__(CM >= 8 ? 256 : 1024) void k_post_walk(PassArgs<R> a, const R* __restrict__ mats,
                                                     const int4* __restrict__ prog, int nprog,
                                                     int ne, int W) {
+  static_assert(NRS == 0 || NRS == kWalkRegDepths, "register depths of the drop bits");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int C = EXACT ? CM : a.C;
   const int wave = uni(threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -364,10 +377,22 @@ __global__ __launch_bounds__(CM >= 8 ? 256 : 1024) void k_post_walk(PassArgs<R> 
     if (last) break;
     // the next entry's stacked child, before this node's stores
     readback(stacked_of(f0, f1), xs);
-    R* p = base + (size_t)(v - a.T) * sstride;
+    if constexpr (sizeof(R) == 4) {
+      // this node's C rows of this sample as one raw buffer (empty for a dropped cherry)
+      const bool drop = a.drop_mode >= 0 && ((e1.w >> a.drop_mode) & 1);
+      const R* node = a.partials + (size_t)(v - a.T) * sstride + (size_t)k * C * cstride;
+      const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+          const_cast<R*>(node), (short)0, drop ? 0 : (int)((size_t)C * cstride * sizeof(R)),
+          0x00020000);
 #pragma unroll
-    for (int c = 0; c < CM; ++c)
-      if (EXACT || c < C) st4(p + (size_t)c * cstride, y[c]);
+      for (int c = 0; c < CM; ++c)
+        if (EXACT || c < C) store_row_f32(rs, (unsigned)((c * cstride + (size_t)n * 4) * sizeof(R)), y[c]);
+    } else {
+      R* p = base + (size_t)(v - a.T) * sstride;
+#pragma unroll
+      for (int c = 0; c < CM; ++c)
+        if (EXACT || c < C) st4(p + (size_t)c * cstride, y[c]);
+    }
     est[so * 64 + lane] = E;
     // only a result that will wait on the stack (the next entry does not
     // consume it) needs its register copy
@@ -697,10 +722,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
   };
   const cptr<R> mk = to_const(mats) + (size_t)k * a.B * C * 16;
   const cptr<int> cprog = to_const(prog);
-  auto entry = [&](int i, int4& f0, int4& f1) {
+  auto entry = [&](int i, int4& f0, int4& f1, int4& f2, int4& f3) {
     const cptr<int> q = cprog + (size_t)kPreInts * i;
     f0.x = q[0], f0.y = q[1], f0.z = q[2], f0.w = q[3];
     f1.x = q[4], f1.y = q[5], f1.z = q[6], f1.w = q[7];
+    f2.x = q[8], f2.y = q[9], f2.z = q[10], f2.w = q[11];
+    f3.x = q[12], f3.y = q[13], f3.z = q[14], f3.w = q[15];
+  };
+  // child h of an entry is a cherry whose partial the post-order walk did not store
+  const int dmode = TIPS == HX_TIPS_CODES ? a.drop_mode : -1;
+  auto dropped = [&](const int4& f1, int h) {
+    return dmode >= 0 && ((f1.w >> (2 * h + dmode)) & 1);
   };
   // Every entry issues the same vector loads (placeholders where the entry
   // has nothing to read), so the wait for one entry's data counts the next
@@ -709,8 +741,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
   // with nothing to read uses the hot line with a category stride of zero.
   const R* wbase = a.partials + (size_t)k * C * cstride + (size_t)tl * 256;
   const size_t sstride = (size_t)a.K * C * cstride;
-  const uint8_t* cbase = a.codes_pad + (size_t)tl * 64 + lane;
-  auto fetch = [&](const int4& f0, const int4& f1, PreIn<CM>& in) {
+  // 16-bit codes: a tip's code, or a dropped cherry's two codes (rows named by the entry)
+  const uint16_t* cbase = a.pcodes + (size_t)tl * 64 + lane;
+  auto fetch = [&](const int4& f0, const int4& f1, const int4& f2, PreIn<CM>& in) {
     if constexpr (TAU) {
       // issued first: the copy of this value into the loop-carried register
       // then waits for this load only
@@ -732,20 +765,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
     for (int h = 0; h < 2; ++h) {
       const int ch = h ? f0.z : f0.y;
       const bool tip = (f1.z >> h) & 1;
-      const R* pb = tip ? hot : wbase + (size_t)(ch - a.T) * sstride;
-      const size_t cs = tip ? 0 : cstride;
-      const unsigned off = tip ? off_hot : off_real;
+      const bool coded = tip || dropped(f1, h);  // from codes: nothing to read here
+      const R* pb = coded ? hot : wbase + (size_t)(ch - a.T) * sstride;
+      const size_t cs = coded ? 0 : cstride;
+      const unsigned off = coded ? off_hot : off_real;
+      // the code first: a tip or a rebuilt cherry waits for it alone
+      if constexpr (TIPS == HX_TIPS_CODES) {
+        in.code[h] = cbase[(size_t)(coded ? (h ? f2.y : f2.x) : 0) * a.np];
+      } else {
+        const int nn = n < a.N ? n : a.N - 1;
+        in.tipv[h] = ld4(tip ? a.dense + ((size_t)ch * a.N + nn) * 4 : at(hot, off_hot));
+      }
       // a post-order partial is read once: streaming, so that pushed
       // adjoints stay in L2 until their pop
 #pragma unroll
       for (int c = 0; c < CM; ++c)
         if (EXACT || c < C) in.L[h][c] = ld4_nt(at(pb + c * cs, off));
-      if constexpr (TIPS == HX_TIPS_CODES) {
-        in.code[h] = cbase[(size_t)(tip ? ch : 0) * a.np];
-      } else {
-        const int nn = n < a.N ? n : a.N - 1;
-        in.tipv[h] = ld4(tip ? a.dense + ((size_t)ch * a.N + nn) * 4 : at(hot, off_hot));
-      }
     }
   };
 
@@ -785,15 +820,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
     }
   };
   PreIn<CM> cur;
-  int4 e0, e1;
-  entry(0, e0, e1);
-  fetch(e0, e1, cur);
+  int4 e0, e1, e2, e3;
+  entry(0, e0, e1, e2, e3);
+  fetch(e0, e1, e2, cur);
   // the store every trip ends with, here a zero that entry 0 overwrites, so
-  // the loop is entered with the same outstanding memory operations
+  // the loop is entered with the same outstanding memory operations (in the
+  // same order: no load may be scheduled after it)
+  __builtin_amdgcn_sched_barrier(0);
   stat_store(e0, 0.f, 0.f);
   for (int i = 0; i < nprog; ++i) {
-    int4 f0 = e0, f1 = e1;
-    if (i + 1 < nprog) entry(i + 1, f0, f1);
+    int4 f0 = e0, f1 = e1, f2 = e2, f3 = e3;
+    if (i + 1 < nprog) entry(i + 1, f0, f1, f2, f3);
     const int ch[2] = {e0.y, e0.z};
     const int usrc = e0.w, cont = e1.x, push = e1.y, tips = e1.z;
 
@@ -831,6 +868,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
           }
         }
       } else {
+        if constexpr (TIPS == HX_TIPS_CODES) {
+          if (dropped(e1, h)) {
+            // the cherry's partial before the post-order walk's power-of-two rescale: every
+            // term below is a ratio in it, so the result is the same as from the stored rows
+            const int code = cur.code[h];
+            const V4<R> t0 = tbl[code & 0xff], t1 = tbl[code >> 8];
+            const int g0 = h ? e3.x : e2.z, g1 = h ? e3.y : e2.w;
+#pragma unroll
+            for (int c = 0; c < CM; ++c) {
+              if (EXACT || c < C) {
+                R P[16];
+                load_p16(mk + ((size_t)g0 * C + c) * 16, P);
+                const V4<R> y0 = mv(P, t0);
+                load_p16(mk + ((size_t)g1 * C + c) * 16, P);
+                const V4<R> y1 = mv(P, t1);
+                // through the placeholder rows just loaded (finite P values, times 0): the
+                // rows are then awaited one category at a time, as on the stored path, and
+                // the two paths join with the same outstanding loads
+#pragma unroll
+                for (int s = 0; s < 4; ++s)
+                  cur.L[h][c].x[s] = fmaf(0.f, cur.L[h][c].x[s], y0.x[s] * y1.x[s]);
+              }
+            }
+          }
+        }
 #pragma unroll
         for (int c = 0; c < CM; ++c) {
           if (EXACT || c < C) {
@@ -952,7 +1014,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
     else if (cont == 1) adjoint(H1{}, U);
 
     // ------------------------------------------------ the next entry's loads
-    fetch(f0, f1, cur);
+    fetch(f0, f1, f2, cur);
 
     // ---------------------------------------------------------------- part 2
     gacc[1] = stat_mfma(1);
@@ -976,6 +1038,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
     }
     e0 = f0;
     e1 = f1;
+    e2 = f2;
+    e3 = f3;
   }
 }
 
@@ -1030,6 +1094,11 @@ int launch_post_dna(const PassArgs<R>& a, const int32_t* prog, const int32_t* se
   const size_t lds = dna_walk_lds<R>(a.T, W, need, tips, a.ncodes);
   HX_REQUIRE(lds <= kDnaWalkLdsMax, HX_ERR_UNSUPPORTED,
              "post-order walk: tip codes of the tile exceed LDS (too many taxa)");
+  HX_REQUIRE(sizeof(R) == 8 || (size_t)a.C * a.np * 4 * sizeof(R) < ((size_t)1 << 31),
+             HX_ERR_UNSUPPORTED, "post-order walk: a node's rows of one sample exceed 2 GiB");
+  HX_REQUIRE(a.drop_mode < 0 || (sizeof(R) == 4 && tips == HX_TIPS_CODES &&
+                                 (a.drop_mode == 0 || a.C == 1 || a.C == 2 || a.C == 4)),
+             HX_ERR_UNSUPPORTED, "post-order walk: drop mode without the register depths it assumes");
   if (tips == HX_TIPS_CODES) {
     const size_t npad = (size_t)a.ntiles * 64, total = (size_t)a.T * npad;
     k_pad_codes<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(a.codes, a.codes_pad, a.T, a.N,
@@ -1096,11 +1165,38 @@ int launch_pre_dna(const PassArgs<R>& a, const int32_t* nodes, int width, int ti
   return HX_OK;
 }
 
+// The pre-order walk's 16-bit codes [T + ncher][np]: row t < T is tip t's code, row T + j
+// cherry j's two codes (first tip's | second's << 8); padding patterns are 0.
+__global__ void k_pair_codes(const uint8_t* __restrict__ codes, const int32_t* __restrict__ cher,
+                             uint16_t* __restrict__ out, int T, int ncher, int N, int np) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)(T + ncher) * np) return;
+  const int r = (int)(idx / np), j = (int)(idx % np);
+  uint16_t v = 0;
+  if (j < N) {
+    if (r < T) {
+      v = codes[(size_t)r * N + j];
+    } else {
+      const int t0 = cher[2 * (r - T)], t1 = cher[2 * (r - T) + 1];
+      v = (uint16_t)(codes[(size_t)t0 * N + j] | (codes[(size_t)t1 * N + j] << 8));
+    }
+  }
+  out[idx] = v;
+}
+
 int launch_pre_walk(const PassArgs<float>& a, const int32_t* prog, int nprog, int tips,
                     hipStream_t st) {
   const int W = kPreWalkW;
   HX_REQUIRE(a.C <= kPreWalkMaxC && a.nchunks == (a.ntiles + W - 1) / W, HX_ERR_UNSUPPORTED,
              "pre-order walk: chunking does not match the workgroup's tiles");
+  if (tips == HX_TIPS_CODES) {
+    HX_REQUIRE(a.pcodes != nullptr && 2 * a.ncher <= a.T, HX_ERR_UNSUPPORTED,
+               "pre-order walk: no room for the 16-bit codes");
+    const size_t total = (size_t)(a.T + a.ncher) * a.np;
+    k_pair_codes<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(a.codes, a.cher, a.pcodes, a.T,
+                                                                  a.ncher, a.N, a.np);
+    HX_HIP_TRY(hipGetLastError());
+  }
   const size_t lds = (size_t)W * 2 * 16 * kXS * sizeof(float) + (size_t)4 * W * 64 * sizeof(float) +
                      tbl_bytes(tips, a.ncodes, sizeof(float));
   HX_REQUIRE(lds <= kDnaWalkLdsMax, HX_ERR_UNSUPPORTED, "pre-order walk: LDS");

@@ -19,6 +19,12 @@
 //   [6] kinds = kind(left) | kind(right) << 8 with kind 0 = tip, 1 = the
 //       result of the previous entry, 2 = an earlier result (stacked: the
 //       first-evaluated child of a node whose children are both internal)
+//   [7] drop  = bit m set when this entry's node is a cherry (both children
+//       tips) whose result its parent takes from registers under register
+//       mode m (0: no stack depth in registers; 1: depths kWalkRegDepth0 ..
+//       kWalkRegDepth0 + kWalkRegDepths - 1 in registers): the partial need
+//       not be stored, because the pre-order walk rebuilds it from the two
+//       tip codes
 // Tips are consumed in walk order; consecutive entries are grouped into
 // segments of at most kWalkSegTips tips so a kernel can stage the codes of
 // one segment while it works through the previous one. segtab holds
@@ -33,11 +39,16 @@ namespace hx {
 
 constexpr int kWalkInts = 8;
 constexpr int kWalkSegTips = 16;
+// Stack depths the post-order walk may keep in registers (register mode 1).
+constexpr int kWalkRegDepth0 = 2;
+constexpr int kWalkRegDepths = 3;
 
 // kids: [2 * nn] children of each node (tips 0..T-1 have -1); postorder:
-// [I][3] triples, root last. Returns false if the program is malformed.
+// [I][3] triples, root last. drop (optional): per node, the [7] bits of its
+// entry. Returns false if the program is malformed.
 inline bool build_walk(int T, const std::vector<int32_t>& kids, const int32_t* postorder,
-                       std::vector<int32_t>& walk, std::vector<int32_t>& segtab, int& walk_need) {
+                       std::vector<int32_t>& walk, std::vector<int32_t>& segtab, int& walk_need,
+                       std::vector<uint8_t>* drop = nullptr) {
   const int I = T - 1, nn = 2 * T - 1;
   const int root = postorder[3 * (I - 1)];
   std::vector<int> need(nn, 0);
@@ -116,8 +127,44 @@ inline bool build_walk(int T, const std::vector<int32_t>& kids, const int32_t* p
     walk_need = std::max(walk_need, f.base + 1 + (second >= 0 ? 1 : 0));
     fs.pop_back();
   }
+  // drop bits: a cherry taken by its parent as the previous result (both
+  // modes) or from a register depth (mode 1)
+  std::vector<uint8_t> bits(nn, 0);
+  std::vector<int> at(nn, -1);
+  for (int i = 0; i < (int)walk.size() / kWalkInts; ++i) at[walk[kWalkInts * i]] = i;
+  for (int i = 0; i < (int)walk.size() / kWalkInts; ++i) {
+    const int32_t* e = walk.data() + (size_t)kWalkInts * i;
+    for (int h = 0; h < 2; ++h) {
+      const int ch = e[1 + h];
+      if (ch < T || kids[2 * ch] >= T || kids[2 * ch + 1] >= T) continue;
+      const int kind = (e[6] >> (8 * h)) & 0xff, slot = ((e[3] >> (8 * h)) & 0xff) - 1;
+      const bool reg = slot >= kWalkRegDepth0 && slot < kWalkRegDepth0 + kWalkRegDepths;
+      bits[ch] = (uint8_t)((kind == 1 ? 1 : 0) | ((kind == 1 || (kind == 2 && reg)) ? 2 : 0));
+    }
+  }
+  for (int i = 0; i < (int)walk.size() / kWalkInts; ++i)
+    walk[(size_t)kWalkInts * i + 7] = bits[walk[(size_t)kWalkInts * i]];
+  if (drop) *drop = bits;
   return (int)walk.size() == kWalkInts * I && walk[kWalkInts * (I - 1)] == root &&
-         walk_need <= 255 && walk_need == need[root] && walk[5] == 0;
+         walk_need <= 255 && walk_need == need[root] && walk[5] == 0 && bits[root] == 0;
+}
+
+// The cherries of a tree (internal nodes whose children are both tips) in
+// post-order, as [ncher][2] tip ids, and the row of each node among them
+// (-1 for other nodes).
+inline void list_cherries(int T, const std::vector<int32_t>& kids, const int32_t* postorder,
+                          std::vector<int32_t>& pairs, std::vector<int32_t>& row) {
+  const int I = T - 1, nn = 2 * T - 1;
+  pairs.clear();
+  row.assign(nn, -1);
+  for (int r = 0; r < I; ++r) {
+    const int v = postorder[3 * r], a = kids[2 * v], b = kids[2 * v + 1];
+    if (a < T && b < T) {
+      row[v] = (int)(pairs.size() / 2);
+      pairs.push_back(a);
+      pairs.push_back(b);
+    }
+  }
 }
 
 
@@ -137,11 +184,21 @@ inline bool build_walk(int T, const std::vector<int32_t>& kids, const int32_t* p
 //   [4] cont = child (0 a, 1 b) whose adjoint the next entry uses, or -1
 //   [5] push = child | d << 8 whose adjoint is pushed at depth d, or -1
 //   [6] tips = (a is a tip) | (b is a tip) << 1
-//   [7] 0
-constexpr int kPreInts = 8;
+//   [7] drop = the post-order drop bits (build_walk) of a | those of b << 2:
+//       under register mode m, child h's partial was not stored when bit
+//       2 h + m is set, and the walk rebuilds it from its tips
+//   [8], [9]   code rows of a and b: a tip's id, a cherry's T + its index
+//              in list_cherries, otherwise 0
+//   [10], [11] a cherry a's two tips (else 0); [12], [13] those of b
+//   [14], [15] 0
+constexpr int kPreInts = 16;
 
+// drop: per node, the bits build_walk reports (nullptr: none dropped);
+// cher_row: list_cherries' rows (nullptr: no cherry rows).
 inline bool build_pre_walk(int T, const std::vector<int32_t>& kids, const int32_t* postorder,
-                           std::vector<int32_t>& prog, int& depth) {
+                           std::vector<int32_t>& prog, int& depth,
+                           const std::vector<uint8_t>* drop = nullptr,
+                           const std::vector<int32_t>* cher_row = nullptr) {
   const int I = T - 1, nn = 2 * T - 1;
   const int root = postorder[3 * (I - 1)];
   std::vector<int> size(nn, 0);  // internal nodes in the subtree
@@ -168,7 +225,17 @@ inline bool build_pre_walk(int T, const std::vector<int32_t>& kids, const int32_
     } else if (ib) {
       cont = 1;
     }
-    const int32_t e[kPreInts] = {cur, a, b, usrc, cont, push, (a < T ? 1 : 0) | (b < T ? 2 : 0), 0};
+    auto dbits = [&](int ch) { return (ch >= T && drop) ? (int)((*drop)[ch] & 3) : 0; };
+    auto crow = [&](int ch) {
+      if (ch < T) return ch;
+      return (cher_row && (*cher_row)[ch] >= 0) ? T + (*cher_row)[ch] : 0;
+    };
+    auto gtip = [&](int ch, int q) {
+      return (ch >= T && cher_row && (*cher_row)[ch] >= 0) ? kids[2 * ch + q] : 0;
+    };
+    const int32_t e[kPreInts] = {cur, a, b, usrc, cont, push, (a < T ? 1 : 0) | (b < T ? 2 : 0),
+                                 dbits(a) | (dbits(b) << 2), crow(a), crow(b), gtip(a, 0),
+                                 gtip(a, 1), gtip(b, 0), gtip(b, 1), 0, 0};
     prog.insert(prog.end(), e, e + kPreInts);
     if (cont >= 0) {
       cur = cont == 0 ? a : b;
