@@ -66,40 +66,7 @@ __global__ void k_dbls(int K, int B, int C, const double* __restrict__ rates,
   dbls[idx] = t;
 }
 
-struct JobWs {
-  size_t mats, mats_dt, prune, prune_bytes, loglik_tmp, dtau_tmp, dtau_acc, total;
-};
-
-// Workspace of one evaluation: P (and dP/dtau), one pattern slice's pruning
-// workspace, and the per-request sums.
-static JobWs job_ws_shape(int taxa, int S, int slice, int C, int K, int dtype, int grad) {
-  const size_t rb = dtype == HX_F64 ? 8 : 4;
-  const size_t B = 2 * (size_t)taxa - 2;
-  const size_t nm = (size_t)K * B * C * S * S;
-  const size_t nt = (size_t)K * B * C;
-  JobWs w{};
-  size_t off = 0;
-  w.mats = off;
-  off = align_up(off + nm * rb, 256);
-  if (grad) {
-    w.mats_dt = off;
-    off = align_up(off + nm * rb, 256);
-  }
-  w.prune = off;
-  w.prune_bytes = workspace_bytes_for(taxa, S, slice, C, K, dtype, grad ? HX_GRAD_TAU : HX_GRAD_NONE);
-  off = align_up(off + w.prune_bytes, 256);
-  w.loglik_tmp = off;
-  off = align_up(off + (size_t)K * 8, 256);
-  if (grad) {
-    w.dtau_tmp = off;
-    off = align_up(off + nt * 8, 256);
-    w.dtau_acc = off;
-    off = align_up(off + nt * 8, 256);
-  }
-  w.total = off;
-  return w;
-}
-
+// (JobWs and job_ws_shape: layout.hpp)
 static JobWs job_ws(const hx_job_s* j, int K, int grad) {
   return job_ws_shape(j->plan->T, j->S, j->slice, j->C, K, j->dtype, grad);
 }

@@ -132,67 +132,10 @@ __global__ __launch_bounds__(256) void k_pad_mats(const R* __restrict__ mats, R*
 }
 
 // ------------------------------------------------------------------- host side
-struct WsLayout {
-  size_t partials, tail, scales, rootsums, gpart, ppad, pcodes, total;  // tail: padding after the partials
-};
-
-// FP32 gradients at 33..64 states read P and P^T from zero-padded 64 x 64 copies.
-static bool use_ppad(int S, int dtype, int grad_mode) {
-  return grad_mode != 0 && dtype != HX_F64 && S > 32 && S <= 64;
-}
-
-// grad_mode: 0 loglik only, 1 d loglik / d mats, 2 d loglik / d tau.
-// The layout depends on the tree only through its taxon count.
-static WsLayout ws_layout_t(int taxa, int S, int N, int C, int K, int dtype, int grad_mode) {
-  const size_t rb = dtype == HX_F64 ? 8 : 4;
-  const Tiling t = make_tiling(N, C, K, S, (int)rb);
-  const size_t I = (size_t)taxa - 1, B = 2 * (size_t)taxa - 2;
-  WsLayout w;
-  size_t off = 0;
-  w.partials = off;
-  const size_t np = S == 4 ? (size_t)t.ntiles * 64 : (size_t)N;
-  // dense states: the pre-order reads up to 16 C S + 64 elements past the
-  // last (node, sample) slab (rows of the last tile past N, padded states)
-  w.tail = off + I * K * np * C * S * rb;
-  const size_t tail = S == 4 ? 0 : (16 * (size_t)C * S + 64) * rb;
-  off = align_up(w.tail + tail, 256);
-  w.scales = off;  // S = 4: 64-byte-aligned copy of the tip codes [T][ntiles * 64]
-  if (S == 4) off = align_up(off + (size_t)taxa * t.ntiles * 64, 256);
-  else off = align_up(off + I * K * (size_t)N * 4, 256);
-  w.rootsums = off;
-  off = align_up(off + (size_t)K * t.ntiles * (1 + S + C) * 8, 256);
-  w.gpart = off;
-  if (grad_mode == 1) off = align_up(off + B * K * (size_t)t.nchunks * C * S * S * rb, 256);
-  if (grad_mode == 2) off = align_up(off + B * K * (size_t)t.nchunks * C * 8, 256);
-  w.ppad = off;
-  if (use_ppad(S, dtype, grad_mode)) off = align_up(off + B * K * (size_t)C * 2 * 64 * 64 * rb, 256);
-  // four-state FP32: 16-bit tip codes and cherry code pairs for the pre-order walk (a tree
-  // has at most T / 2 cherries)
-  w.pcodes = off;
-  if (S == 4 && rb == 4)
-    off = align_up(off + ((size_t)taxa + (size_t)taxa / 2) * t.ntiles * 64 * 2, 256);
-  w.total = off;
-  return w;
-}
-
-// Register mode of the post-order walk's drop bits (walk.hpp) for a call, or -1 when every
-// internal partial is stored: cherries are dropped for four-state FP32 calls with tip codes
-// whose gradient runs as the pre-order walk (use_pre_walk), the only reader that rebuilds
-// them. Mode 1 where the post-order walk keeps stack depths in registers (FP32 at 1, 2 or 4
-// categories), else mode 0.
-static int walk_drop_mode(int S, int C, int N, int K, int dtype, int tip_kind) {
-  if (S != 4 || dtype == HX_F64 || tip_kind != HX_TIPS_CODES || !use_pre_walk(S, C, 4, N, K))
-    return -1;
-  return (C == 1 || C == 2 || C == 4) ? 1 : 0;
-}
-
+// (tiling and workspace layout: layout.hpp)
 static WsLayout ws_layout(const hx_plan_s* plan, int S, int N, int C, int K, int dtype,
                           int grad_mode) {
   return ws_layout_t(plan->T, S, N, C, K, dtype, grad_mode);
-}
-
-size_t workspace_bytes_for(int taxa, int S, int N, int C, int K, int dtype, int grad_mode) {
-  return ws_layout_t(taxa, S, N, C, K, dtype, grad_mode).total;
 }
 
 template <typename R>

@@ -53,7 +53,6 @@ __device__ __forceinline__ void lds_barrier() {
 }
 
 // Dense-state (S != 4) launchers, defined in pruning_wide.hip.
-int wide_nt(int C, int S, int rb);
 template <typename R>
 int launch_post_wide(const PassArgs<R>& a, const int32_t* nodes, int width, int tips, bool root,
                      int S, hipStream_t st);
@@ -77,8 +76,7 @@ int launch_pre_walk(const PassArgs<float>& a, const int32_t* prog, int nprog, in
                     hipStream_t st);
 
 // Dense-state kernels on MFMA (pruning_mfma.hip, FP32 and FP64); used when
-// use_mfma().
-bool use_mfma(int C, int S, int dtype);
+// use_mfma() (layout.hpp).
 template <typename R>
 int launch_mfma(bool post, const PassArgs<R>& a, const int32_t* nodes, int width, int tips,
                 bool root, int S, hipStream_t st);

@@ -1194,10 +1194,6 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
   }
 }
 
-bool use_mfma(int C, int S, int dtype) {
-  return (dtype == HX_F32 || dtype == HX_F64) && S != 4 && C >= 1 && C <= 4;
-}
-
 template <typename R>
 static size_t mfma_lds(int C, int S, bool pre, int tiles_per_chunk) {
   const int MT = S <= 16 ? 1 : (S <= 32 ? 2 : 4);

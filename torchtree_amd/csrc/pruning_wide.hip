@@ -21,7 +21,6 @@ namespace hx {
 namespace {
 
 constexpr int kWideMaxThreads = 512;
-constexpr int kWideLdsBytes = 60 * 1024;
 
 __device__ __forceinline__ float wfmax(float a, float b) { return fmaxf(a, b); }
 __device__ __forceinline__ double wfmax(double a, double b) { return fmax(a, b); }
@@ -80,15 +79,6 @@ __device__ __forceinline__ void fill_children(const PassArgs<R>& a, R* Lt, int c
 
 }  // namespace
 
-int wide_nt(int C, int S, int rb) {
-  if (use_mfma(C, S, rb == 8 ? HX_F64 : HX_F32)) return 16;  // MFMA tiles of 16 patterns
-  const int SP = S <= 32 ? 32 : 64;
-  const int G = C * SP;
-  int nt = kWideLdsBytes / (7 * G * rb);
-  if (nt > 8) nt = 8;
-  if (nt < 1) nt = 1;
-  return nt;
-}
 
 // ---------------------------------------------------------------- post-order
 template <typename R, int SP, int TIPS, bool ROOT>
