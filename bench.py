@@ -62,6 +62,9 @@ def parse():
     ap.add_argument("--serve-plain-limit", type=float, default=150.0,
                     help="seconds of sequential plain FP64 evaluation (stats over completed requests)")
     ap.add_argument("--serve-streams", type=int, default=8)
+    ap.add_argument("--serve-dense-products", default="bf16x3", choices=("fp32", "bf16x3"),
+                    help="products of the served 20- and 61-state jobs (the four-state jobs "
+                         "have no dense products)")
     ap.add_argument("--serve-graph-bytes", type=int, default=64 << 20,
                     help="requests with at most this much workspace replay a captured HIP graph "
                          "(0: every request launches eagerly)")
@@ -332,7 +335,8 @@ def serving_bench(args, dev):
                   sp["rates"], sp["props"])
         j32 = serving.ResidentJob(sp["name"], *common, dtype=torch.float32,
                                   slice_patterns=srv.slice_for(T, N, C, S, torch.float32),
-                                  device=dev.index or 0, kind=sp["kind"])
+                                  device=dev.index or 0, kind=sp["kind"],
+                                  products=args.serve_dense_products if S != 4 else "fp32")
         srv.register(sp["name"], j32)
         jobs32.append(j32)
         jobs64.append(serving.ResidentJob(sp["name"], *common, dtype=torch.float64,
@@ -378,7 +382,8 @@ def serving_bench(args, dev):
         "workload": f"{len(specs)} resident jobs (T~U{{16..1024}}, N log-uniform 100-100000, "
                     f"DNA/AA/codon {kinds.count('dna')}/{kinds.count('aa')}/{kinds.count('codon')} "
                     f"requests), K in {{1,8,32}}, Poisson arrivals at {rate:.1f}/s (80% of capacity), "
-                    f"{args.serve_streams} streams, {srv.pool.size >> 30} GiB workspace pool",
+                    f"{args.serve_streams} streams, {srv.pool.size >> 30} GiB workspace pool, "
+                    f"{args.serve_dense_products} products for the 20- and 61-state jobs",
         "requests": st["requests"],
         "served_updates_per_s": st["updates_per_s"],
         "served_p50_ms": st["p50_ms"], "served_p99_ms": st["p99_ms"],

@@ -72,9 +72,12 @@ class Plan:
         return int(_lib.load().hx_workspace_bytes(self.handle, S, N, C, K, dtype_code, grad_mode))
 
     def __del__(self):
+        # at interpreter exit the module's globals may already be gone: the device memory
+        # then goes with the process
         h = getattr(self, "handle", None)
-        if h is not None and _lib._lib is not None:
-            _lib._lib.hx_plan_destroy(h)
+        lib = getattr(_lib, "_lib", None) if _lib is not None else None
+        if h is not None and lib is not None:
+            lib.hx_plan_destroy(h)
             self.handle = None
 
 
