@@ -500,50 +500,36 @@ __global__ __launch_bounds__(512) void k_post_mfma_r(PassArgs<R> a,
   const int chq = ((lane >> 4) & 1) ? ch1 : ch0;
   const int32_t* sq = a.scales + ((size_t)(chq >= a.T ? chq - a.T : 0) * a.K + k) * a.N;
   const float qmask = chq >= a.T ? 1.f : 0.f;
-  // a tile's child rows and the children's scales of its patterns (one tile ahead: each
-  // tile issues the next tile's loads after its product, so their latency overlaps the
-  // tile's barriers, exchanges and stores)
-  // The values are kept as loaded and masked when the tile uses them, so that nothing waits
-  // for the loads before then.
-  auto rows = [&](int tile, R (&raw)[KV], int& sv) {
-    const int nbase = tile * kTile;
-    const int ncl = min(nbase + nn, a.N - 1);
-    const R* bp;
-    uint32_t base, last;  // first element and last state of the row read
-    if (internal) {
-      bp = cslab, base = (uint32_t)(ncl * CS + c * S);
-    } else if constexpr (TIPS == HX_TIPS_CODES) {
-      bp = a.table, base = (uint32_t)(mycodes[(tile - t0) * kTile + nn] * S);
-    } else {
-      bp = a.dense + (size_t)chh * a.N * S, base = (uint32_t)(ncl * S);
-    }
-    last = base + S - 1;
-    base += g * KV;
-#pragma unroll
-    for (int q = 0; q < KV; ++q) raw[q] = bp[min(base + q, last)];
-    sv = sq[min(nbase + (lane & 15), a.N - 1)];
-  };
   lds_barrier();
-  R raw[KV];
-  int sraw;
-  rows(t0, raw, sraw);
 
   for (int tile = t0; tile < t1; ++tile) {
     const int nbase = tile * kTile;
     const uint32_t tb = (uint32_t)nbase * (uint32_t)CS;
     R arow[KV];
     {
-      const R rowm = nbase + nn < a.N ? R(1) : R(0);
+      const int n = nbase + nn;
+      const int ncl = min(n, a.N - 1);
+      const R rowm = n < a.N ? R(1) : R(0);
+      const R* bp;
+      uint32_t base, last;  // first element and last state of the row read
+      if (internal) {
+        bp = cslab, base = (uint32_t)(ncl * CS + c * S);
+      } else if constexpr (TIPS == HX_TIPS_CODES) {
+        bp = a.table, base = (uint32_t)(mycodes[(tile - t0) * kTile + nn] * S);
+      } else {
+        bp = a.dense + (size_t)chh * a.N * S, base = (uint32_t)(ncl * S);
+      }
+      last = base + S - 1;
+      base += g * KV;
 #pragma unroll
-      for (int q = 0; q < KV; ++q) arow[q] = raw[q] * ((g * KV + q < S) ? rowm : R(0));
+      for (int q = 0; q < KV; ++q)
+        arow[q] = bp[min(base + q, last)] * ((g * KV + q < S) ? rowm : R(0));
     }
-    const float svt = (float)sraw * qmask;
+    const float sv = (float)sq[min(nbase + (lane & 15), a.N - 1)] * qmask;
     vec4_t<R> acc[MT];
 #pragma unroll
     for (int t = 0; t < MT; ++t) acc[t] = vec4_t<R>{R(0), R(0), R(0), R(0)};
     fp.product(arow, acc);
-    // the next tile's loads (the last tile re-reads its own, unused)
-    rows(min(tile + 1, t1 - 1), raw, sraw);
     // lane (nn, g): acc[t][r] = Y[m = acc_row(g, r)][i = 16 t + nn]
     if (h == 1) {
 #pragma unroll
@@ -552,7 +538,7 @@ __global__ __launch_bounds__(512) void k_post_mfma_r(PassArgs<R> a,
         for (int r = 0; r < 4; ++r) Yx[(c * kTile + acc_row<R>(g, r)) * SP + 16 * t + nn] = acc[t][r];
     }
     int* et = Et + (tile & 1) * 2 * kTile;
-    if (wave == 0 && lane < 2 * kTile) et[lane] = (int)svt;
+    if (wave == 0 && lane < 2 * kTile) et[lane] = (int)sv;
     lds_barrier();
     R x[MT][4];
     if (h == 0) {
