@@ -91,19 +91,38 @@ def walk_need(post, T):
 
 
 def pre_walk_used(S, C, rb, N, K):
-    """The four-state FP32 pre-order runs as one walk (mirrors use_pre_walk, csrc/common.hpp)."""
+    """The four-state FP32 pre-order runs as one walk (mirrors use_pre_walk, csrc/layout.hpp)."""
     groups = ((N + 63) // 64 + 3) // 4 * K
     return S == 4 and rb == 4 and C <= 4 and groups >= 256
 
 
-def algorithmic_bytes(T, N, C, K, S, rb, nchunks):
+def walk_drops(T, post, dev, S, C, N, K, rb):
+    """Cherries whose partial the four-state FP32 walks neither store nor read back (the
+    post-order walk drops them, the pre-order walk rebuilds them from two tip codes);
+    mirrors walk_drop_mode in csrc/layout.hpp."""
+    if not (S == 4 and pre_walk_used(S, C, rb, N, K)):
+        return 0
+    import ctypes
+
+    from torchtree_amd import _lib
+    from torchtree_amd.likelihood import get_plan
+
+    plan = get_plan(T, post, dev.index or 0)
+    d0, d1 = ctypes.c_int(), ctypes.c_int()
+    _lib.check(_lib.load().hx_plan_walk_info(plan.handle, None, None, None, ctypes.byref(d0),
+                                             ctypes.byref(d1)))
+    return d1.value if C in (1, 2, 4) else d0.value
+
+
+def algorithmic_bytes(T, N, C, K, S, rb, nchunks, drops=0):
     """Bytes each pass must move at minimum with this algorithm (one launch set).
 
     P = K N C S rb is one node's partials. Four states: the post-order walk
-    writes every internal non-root partial once and reads none back from HBM
-    (a child that waited on the stack is re-read from L2), plus the tip codes.
-    The four-state FP32 pre-order walk reads every internal non-root partial
-    once (the adjoints live in registers), reads the tip codes once per sample
+    writes every internal non-root partial once, except the ``drops`` cherries
+    it does not store, and reads none back from HBM (a child that waited on the
+    stack is re-read from L2), plus the tip codes. The four-state FP32 pre-order
+    walk reads the same partials once (the adjoints live in registers), reads
+    each tip's 16-bit code once per sample and each dropped cherry's code pair,
     and writes the per-chunk branch statistics, which the reduction reads
     back. Otherwise the pre-order runs per level: each internal non-root node's
     L and its parent's U are read and its U written (3 P per node). Other state
@@ -117,14 +136,26 @@ def algorithmic_bytes(T, N, C, K, S, rb, nchunks):
     weights = N * 8
     gpart = (2 * T - 2) * K * nchunks * C * S * S * rb
     if S == 4:
-        post_b = (I - 1) * part + tips + mats + weights
+        post_b = (I - 1 - drops) * part + tips + mats + weights
     else:
         post_b = 2 * (I - 1) * part + 2 * (I - 1) * K * N * 4 + tips + mats + weights
     if pre_walk_used(S, C, rb, N, K):
-        pre_b = (I - 1) * part + K * tips + mats + weights + 2 * gpart
+        pre_b = (I - 1 - drops) * part + K * (T + drops) * N * 2 + mats + weights + 2 * gpart
     else:
         pre_b = 3 * (I - 1) * part + tips + mats + weights + gpart
     return post_b, pre_b
+
+
+def canonical_bytes(T, N, C, K, S, rb, tip_bytes=1):
+    """Bytes per pass of the level-by-level method, per update: the post-order reads two
+    children and writes the parent plus a 4-byte scale per pattern and sample,
+    N C K S b [(T-1) + (T-2)] + T N tau + (T-1) N K 4; the pre-order reads the parent's
+    adjoint and the sibling's partial, writes the child's adjoint and reads its partial for
+    the branch term, (2T-2) N C K 4 S b."""
+    part = N * C * K * S * rb
+    post = part * ((T - 1) + (T - 2)) + T * N * tip_bytes + (T - 1) * N * K * 4
+    pre = (2 * T - 2) * part * 4
+    return {"postorder": post, "preorder": pre}
 
 
 def main():
@@ -214,7 +245,9 @@ def main():
     ll32 = ll32.detach().cpu()
     g32 = (b32.grad.detach().cpu(), e32.grad.detach().cpu(), f32.grad.detach().cpu())
     nchunks = _nchunks(wl.N, wl.C, wl.K)
-    post_b, pre_b = algorithmic_bytes(wl.T, wl.N, wl.C, wl.K, S, 4, nchunks)
+    drops = walk_drops(wl.T, post, dev, S, wl.C, wl.N, wl.K, 4)
+    post_b, pre_b = algorithmic_bytes(wl.T, wl.N, wl.C, wl.K, S, 4, nchunks, drops)
+    canon_b = canonical_bytes(wl.T, wl.N, wl.C, wl.K, S, 4)
     post_ms = phases["postorder"]["ms"]
     pre_ms = phases["preorder"]["ms"]
     post_gbs = post_b / (post_ms * 1e-3) / 1e9
@@ -256,6 +289,12 @@ def main():
                        if dom == "preorder" else "k_post_walk (one launch, whole tree) + k_reduce_root"),
             "achieved": dom_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dom_gbs / HBM_PEAK_GBS,
             "traffic": measured_traffic(args, dom), "algorithmic_bytes": dom_b, "pass_ms": dom_ms,
+            # the measured HBM bytes over the pass time, and the per-update bytes of the
+            # level-by-level method (every adjoint written and read back) over the same time
+            "traffic_frac": (None if measured_traffic(args, dom) is None else
+                             measured_traffic(args, dom) / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS),
+            "canonical_bytes": canon_b[dom],
+            "canonical_frac": canon_b[dom] / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
             # the walks are one launch for the whole tree (pass time includes the
             # small reduction that follows); level kernels one launch per level
             "avg_launch_us": dom_ms * 1e3 / (1 if (dom == "postorder" or pre_walk_used(S, wl.C, 4, wl.N, wl.K))
@@ -263,7 +302,8 @@ def main():
         },
         "other_pass": {"postorder_gbs": post_gbs, "preorder_gbs": pre_gbs,
                        "postorder_bytes": post_b, "preorder_bytes": pre_b,
-                       "walk_stack_entries": walk_need(post, wl.T)},
+                       "walk_stack_entries": walk_need(post, wl.T),
+                       "cherries_not_stored": drops},
     }
 
     # ---- FP64 plain path: accuracy reference at full size, and its throughput
@@ -416,7 +456,7 @@ def measured_traffic(args, dom):
 
 
 def _nchunks(N, C, K, rb=4):
-    """Chunk count of the S=4 kernels (mirrors make_tiling in csrc/common.hpp)."""
+    """Chunk count of the S=4 kernels (mirrors make_tiling in csrc/layout.hpp)."""
     ppb = 64
     ntiles = (N + ppb - 1) // ppb
     if pre_walk_used(4, C, rb, N, K):

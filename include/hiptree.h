@@ -86,6 +86,14 @@ int hx_plan_destroy(hx_plan_t plan);
 /* levels: post-order height levels; depth: pre-order depth levels. */
 int hx_plan_info(hx_plan_t plan, int* post_levels, int* pre_levels, int* max_width);
 
+/* The four-state walk programs of a plan (no reference counterpart: accounting of what
+ * the walks move): post-order stack entries, pre-order stack depth, cherries (internal
+ * nodes whose children are both tips), and the cherries whose partial the post-order walk
+ * does not store under register mode 0 and 1 (rebuilt by the pre-order walk). Any output
+ * pointer may be NULL. */
+int hx_plan_walk_info(hx_plan_t plan, int* post_need, int* pre_depth, int* cherries,
+                      int* dropped_mode0, int* dropped_mode1);
+
 /* Device workspace for one call. grad_mode: HX_GRAD_NONE (hx_loglik only),
  * HX_GRAD_MATS (then hx_loglik_grad_mats), HX_GRAD_TAU (then hx_loglik_grad_tau). */
 enum { HX_GRAD_NONE = 0, HX_GRAD_MATS = 1, HX_GRAD_TAU = 2 };

@@ -346,3 +346,19 @@ def test_site_pattern_indices_match_oracle():
     got = [["".join(chr(c) for c in cols[t, :, j]) for j in range(cols.shape[2])]
            for t in range(6)]
     assert [list(p) for p in ref] == got
+
+
+def test_bench_byte_accounting():
+    """bench.py's roofline bytes: the level-by-level per-update figures reproduce the worked
+    numbers for the 512-taxon, 100,000-pattern, K=32 step (216 GB post-order, 837 GB
+    pre-order); the walks' own minimum drops one partial per unstored cherry from each pass."""
+    import bench
+
+    canon = bench.canonical_bytes(512, 100_000, 4, 32, 4, 4)
+    assert round(canon["postorder"] / 1e9) == 216 and round(canon["preorder"] / 1e9) == 837
+    part = 32 * 100_000 * 4 * 4 * 4
+    post0, pre0 = bench.algorithmic_bytes(512, 100_000, 4, 32, 4, 4, 391, 0)
+    post1, pre1 = bench.algorithmic_bytes(512, 100_000, 4, 32, 4, 4, 391, 171)
+    assert post0 - post1 == 171 * part
+    assert pre0 - pre1 == 171 * part - 32 * 171 * 100_000 * 2  # the cherry code pairs are read
+    assert bench.walk_drops(512, [], None, 20, 4, 100_000, 32, 4) == 0  # dense states drop none

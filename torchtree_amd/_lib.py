@@ -35,6 +35,7 @@ _SIGNATURES = {
     "hx_plan_destroy": (_c_int, [_vp]),
     "hx_plan_info": (_c_int, [_vp, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int),
                               ctypes.POINTER(_c_int)]),
+    "hx_plan_walk_info": (_c_int, [_vp] + [ctypes.POINTER(_c_int)] * 5),
     "hx_workspace_bytes": (_c_size, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int]),
     "hx_loglik": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int,
                            _c_int, _vp, _vp, _c_int,

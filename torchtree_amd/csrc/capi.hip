@@ -161,6 +161,10 @@ int hx_plan_create(int taxa, const int32_t* postorder, int device, hx_plan_t* ou
   plan->prew_depth = prew_depth;
   plan->d_cher = plan->d_buf + cher_off;
   plan->ncher = (int)(cher.size() / 2);
+  for (uint8_t b : drop) {
+    plan->ndrop[0] += b & 1;
+    plan->ndrop[1] += (b >> 1) & 1;
+  }
   *out = plan;
   return HX_OK;
 }
@@ -169,6 +173,17 @@ int hx_plan_destroy(hx_plan_t plan) {
   if (!plan) return HX_OK;
   if (plan->d_buf) hipFree(plan->d_buf);
   delete plan;
+  return HX_OK;
+}
+
+int hx_plan_walk_info(hx_plan_t plan, int* post_need, int* pre_depth, int* cherries,
+                      int* dropped_mode0, int* dropped_mode1) {
+  HX_REQUIRE(plan != nullptr, HX_ERR_INVALID_ARGUMENT, "null plan");
+  if (post_need) *post_need = plan->walk_need;
+  if (pre_depth) *pre_depth = plan->prew_depth;
+  if (cherries) *cherries = plan->ncher;
+  if (dropped_mode0) *dropped_mode0 = plan->ndrop[0];
+  if (dropped_mode1) *dropped_mode1 = plan->ndrop[1];
   return HX_OK;
 }
 

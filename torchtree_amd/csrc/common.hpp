@@ -62,7 +62,9 @@ struct hx_plan_s {
   // internal node; its stack reaches `prew_depth` entries.
   const int32_t* d_prew = nullptr;
   int prew_depth = 0;
-  // Cherries (walk.hpp, list_cherries): [ncher][2] tip ids.
+  // Cherries (walk.hpp, list_cherries): [ncher][2] tip ids; how many the post-order walk
+  // drops under register mode 0 and 1.
   const int32_t* d_cher = nullptr;
   int ncher = 0;
+  int ndrop[2] = {0, 0};
 };
