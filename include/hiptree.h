@@ -217,6 +217,15 @@ int hx_job_create(hx_plan_t plan, int S, int N, int C, int dtype, int slice_patt
                   const double* lambda, const double* left, const double* right,
                   const double* rates, const double* props, const double* freqs, int device,
                   hx_job_t* job);
+/* The same job for a non-reversible model (no eigensystem): Q [S][S] the normalised
+ * general rate matrix (host, double; S <= 32). Requests then form P = exp(Q tau) and
+ * dP/dtau = Q P with hx_pmatrix_expm / hx_pmatrix_expm_dtau, the device form of
+ * NonSymmetricSubstitutionModel.p_t (substitution_model/abstract.py:88-94); everything
+ * else is as for hx_job_create. hx_job_eval_model refuses such a job. */
+int hx_job_create_general(hx_plan_t plan, int S, int N, int C, int dtype, int slice_patterns,
+                          const uint8_t* codes, const double* table, int ncodes,
+                          const double* weights, const double* q, const double* rates,
+                          const double* props, const double* freqs, int device, hx_job_t* job);
 int hx_job_destroy(hx_job_t job);
 size_t hx_job_resident_bytes(hx_job_t job);
 /* Workspace of one hx_job_eval with K samples (with or without the gradient). */

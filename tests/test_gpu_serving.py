@@ -287,3 +287,72 @@ def test_server_model_requests(gpu):
         if a.model is not None:
             for key in ("drates", "dq", "dfreqs", "dprops"):
                 assert torch.equal(a.grads[key], b.grads[key]), (a.rid, key)
+
+
+@pytest.mark.parametrize("dtype,rl,rg", [(F64, 1e-12, 1e-8), (torch.float32, 1e-6, 1e-5)])
+@pytest.mark.parametrize("kind,S", [("dna", 4), ("aa", 20)])
+def test_served_general_model_matches_oracle(gpu, dtype, rl, rg, kind, S):
+    """A resident job with a non-reversible rate matrix (hx_job_create_general: P = exp(Q tau)
+    per request) in two pattern slices, against the oracle with P from torch.matrix_exp in FP64
+    (NonSymmetricSubstitutionModel.p_t, abstract.py:88-94) and autograd for d/dbls; then in a
+    server mix with reversible jobs, bitwise the same alone and in the mix. The FP64 gradient
+    bound is 1e-8: the engine forms dP/dtau as Q P, whose rows cancel (Q's rows sum to zero),
+    against autograd through the block exponential (2e-9 measured at 20 states)."""
+    print(f"case: served general Q {kind} {dtype}", flush=True)
+    from torchtree_amd import serving, synthetic
+
+    rng = np.random.default_rng(5 + S)
+    sp = synthetic.random_job_spec(kind, 18, 700, rng)
+    post = [tuple(int(x) for x in r) for r in sp["topo"].postorder]
+    pi = torch.as_tensor(sp["freqs"], dtype=F64)
+    exch = torch.tensor(rng.gamma(1.5, 1.0, S * (S - 1)), dtype=F64)
+    q = omodels.normalize_q(omodels.general_nonsymmetric_q(exch, torch.arange(S * (S - 1)), pi), pi)
+    jb = serving.ResidentJob("gen", post, sp["codes"], sp["table"], sp["weights"], q, pi,
+                             sp["rates"], sp["props"], dtype=dtype, slice_patterns=400,
+                             device=gpu.index or 0, kind=kind, general=True)
+    K = 3
+    bls = synthetic.draw_bls(sp, K, 21)
+    req = serving.Request(0, jb, bls)
+    serving.run_sequential([req], device=gpu.index or 0)
+    b = bls.clone().requires_grad_(True)
+    tau = b.unsqueeze(-1) * torch.as_tensor(sp["rates"], dtype=F64).reshape(1, 1, -1)
+    mats = torch.matrix_exp(q * tau[..., None, None])
+    tbl = torch.as_tensor(sp["table"])
+    tips = [tbl[torch.as_tensor(c).long()].t().contiguous() for c in sp["codes"]]
+    ref = opruning.treelik_discrete_rescaled(tips + [None] * (len(tips) - 1),
+                                             torch.as_tensor(sp["weights"]), post, mats,
+                                             pi.reshape(1, -1),
+                                             torch.as_tensor(sp["props"]).reshape(-1, 1, 1)).reshape(-1)
+    ref.sum().backward()
+    rel = float(((req.loglik.cpu() - ref.detach()).abs() / ref.detach().abs()).max())
+    assert rel < rl, rel
+    g = b.grad[:, :-1]  # slot 2T-3 is a padded (fixed) length
+    err = float((req.dbls.cpu()[:, :-1] - g).norm() / g.norm())
+    assert err < rg, err
+    # a per-request eigensystem is refused for such a job
+    from torchtree_amd import models as hm
+    with pytest.raises(serving.HipTreeError):
+        jb.eval_model(bls.to(gpu), hm.Eigen(sp["q_norm"], pi, gpu),
+                      torch.ones(1, jb.C, dtype=F64, device=gpu),
+                      torch.full((1, jb.C), 1.0 / jb.C, dtype=F64, device=gpu),
+                      pi.reshape(1, -1).to(gpu), torch.empty(jb.model_workspace_bytes(K) + 256,
+                                                             dtype=torch.uint8, device=gpu),
+                      torch.cuda.current_stream(gpu).cuda_stream)
+    # alone == mix with reversible jobs
+    specs, jobs = _jobs(gpu, dtype)
+    alljobs = jobs + [jb]
+    allspecs = specs + [sp]
+    stream = synthetic.request_stream(24, len(alljobs), rate=3000.0, seed=4, ks=(1, 3))
+    def reqs():
+        return [serving.Request(i, alljobs[j], synthetic.draw_bls(allspecs[j], Kq, s), arrival=t)
+                for i, (j, Kq, s, t) in enumerate(stream)]
+    alone = reqs()
+    serving.run_sequential(alone, device=gpu.index or 0)
+    srv = serving.Server(32 << 30, n_streams=4, pool_bytes=16 << 30, device=gpu.index or 0)
+    for i, jj in enumerate(alljobs):
+        srv.register(f"j{i}", jj)
+    mixed = reqs()
+    srv.run(mixed)
+    assert any(r.job is jb for r in mixed)
+    for a, b2 in zip(alone, mixed):
+        assert torch.equal(a.loglik, b2.loglik) and torch.equal(a.dbls, b2.dbls), a.rid

@@ -67,6 +67,9 @@ _SIGNATURES = {
     "hx_job_create": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int,
                                _vp, _vp, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_int,
                                ctypes.POINTER(_vp)]),
+    "hx_job_create_general": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int,
+                                       _vp, _vp, _c_int, _vp, _vp, _vp, _vp, _vp, _c_int,
+                                       ctypes.POINTER(_vp)]),
     "hx_job_destroy": (_c_int, [_vp]),
     "hx_job_resident_bytes": (_c_size, [_vp]),
     "hx_job_workspace_bytes": (_c_size, [_vp, _c_int, _c_int]),

@@ -31,6 +31,7 @@ struct hx_job_s {
   double* d_lambda = nullptr;      // [S]
   double* d_left = nullptr;        // [S][S]
   double* d_right = nullptr;       // [S][S]
+  bool general = false;            // a general rate matrix (d_left holds Q) instead of an eigensystem
   double* d_rates = nullptr;       // [C]
   void* d_props = nullptr;         // [C] in dtype
   void* d_freqs = nullptr;         // [S] in dtype
@@ -125,17 +126,15 @@ static void cast_rows(const double* src, void* dst, size_t n, int dtype, hipStre
   else k_cast_rows<float><<<g, 256, 0, st>>>(src, static_cast<float*>(dst), n);
 }
 
-}  // namespace hx
-
-extern "C" {
-
-int hx_job_create(hx_plan_t plan, int S, int N, int C, int dtype, int slice_patterns,
-                  const uint8_t* codes, const double* table, int ncodes, const double* weights,
-                  const double* lambda, const double* left, const double* right,
-                  const double* rates, const double* props, const double* freqs, int device,
-                  hx_job_t* out) {
-  HX_REQUIRE(plan && codes && table && weights && lambda && left && right && rates && props &&
-                 freqs && out,
+// A job with the eigensystem (lambda, left, right) of a reversible Q, or with a general Q
+// (q non-null: stored where `left` goes, lambda and right zero).
+static int job_create(hx_plan_t plan, int S, int N, int C, int dtype, int slice_patterns,
+                      const uint8_t* codes, const double* table, int ncodes, const double* weights,
+                      const double* lambda, const double* left, const double* right,
+                      const double* q, const double* rates, const double* props,
+                      const double* freqs, int device, hx_job_t* out) {
+  HX_REQUIRE(plan && codes && table && weights && (q || (lambda && left && right)) && rates &&
+                 props && freqs && out,
              HX_ERR_INVALID_ARGUMENT, "null argument");
   HX_REQUIRE(S >= 2 && S <= 64 && N >= 1 && C >= 1 && ncodes >= 1 && ncodes <= hx::kMaxCodes,
              HX_ERR_INVALID_ARGUMENT, "bad dimensions");
@@ -195,9 +194,13 @@ int hx_job_create(hx_plan_t plan, int S, int N, int C, int dtype, int slice_patt
     if (dtype == HX_F64) reinterpret_cast<double*>(img.data() + o_table)[q] = table[q];
     else reinterpret_cast<float*>(img.data() + o_table)[q] = (float)table[q];
   }
-  std::copy(lambda, lambda + S, reinterpret_cast<double*>(img.data() + o_lam));
-  std::copy(left, left + S * S, reinterpret_cast<double*>(img.data() + o_left));
-  std::copy(right, right + S * S, reinterpret_cast<double*>(img.data() + o_right));
+  if (q) {
+    std::copy(q, q + S * S, reinterpret_cast<double*>(img.data() + o_left));
+  } else {
+    std::copy(lambda, lambda + S, reinterpret_cast<double*>(img.data() + o_lam));
+    std::copy(left, left + S * S, reinterpret_cast<double*>(img.data() + o_left));
+    std::copy(right, right + S * S, reinterpret_cast<double*>(img.data() + o_right));
+  }
   std::copy(rates, rates + C, reinterpret_cast<double*>(img.data() + o_rates));
   for (int q = 0; q < C; ++q) {
     if (dtype == HX_F64) reinterpret_cast<double*>(img.data() + o_props)[q] = props[q];
@@ -228,12 +231,37 @@ int hx_job_create(hx_plan_t plan, int S, int N, int C, int dtype, int slice_patt
   j->d_lambda = reinterpret_cast<double*>(base + o_lam);
   j->d_left = reinterpret_cast<double*>(base + o_left);
   j->d_right = reinterpret_cast<double*>(base + o_right);
+  j->general = q != nullptr;
   j->d_rates = reinterpret_cast<double*>(base + o_rates);
   j->d_props = base + o_props;
   j->d_freqs = base + o_freqs;
   j->resident = off;
   *out = j;
   return HX_OK;
+}
+
+}  // namespace hx
+
+extern "C" {
+
+int hx_job_create(hx_plan_t plan, int S, int N, int C, int dtype, int slice_patterns,
+                  const uint8_t* codes, const double* table, int ncodes, const double* weights,
+                  const double* lambda, const double* left, const double* right,
+                  const double* rates, const double* props, const double* freqs, int device,
+                  hx_job_t* out) {
+  HX_REQUIRE(lambda && left && right, HX_ERR_INVALID_ARGUMENT, "null argument");
+  return hx::job_create(plan, S, N, C, dtype, slice_patterns, codes, table, ncodes, weights,
+                        lambda, left, right, nullptr, rates, props, freqs, device, out);
+}
+
+int hx_job_create_general(hx_plan_t plan, int S, int N, int C, int dtype, int slice_patterns,
+                          const uint8_t* codes, const double* table, int ncodes,
+                          const double* weights, const double* q, const double* rates,
+                          const double* props, const double* freqs, int device, hx_job_t* out) {
+  HX_REQUIRE(q, HX_ERR_INVALID_ARGUMENT, "null argument");
+  HX_REQUIRE(S <= 32, HX_ERR_UNSUPPORTED, "general rate matrices: at most 32 states");
+  return hx::job_create(plan, S, N, C, dtype, slice_patterns, codes, table, ncodes, weights,
+                        nullptr, nullptr, nullptr, q, rates, props, freqs, device, out);
 }
 
 int hx_job_destroy(hx_job_t job) {
@@ -269,12 +297,19 @@ static int job_eval_in(hx_job_t job, int K, const double* bls, double* out_logli
   char* base = static_cast<char*>(workspace);
   void* mats = base + w.mats;
   void* mats_dt = grad ? base + w.mats_dt : nullptr;
-  int rc = hx_pmatrix(job->S, K, job->B, job->C, 0, job->d_lambda, job->d_left, job->d_right, bls,
-                      job->d_rates, 0, mats, job->dtype, stream);
+  // P and dP/dtau: from the eigensystem, or by exp(Q tau) for a general Q
+  int rc = job->general
+               ? hx_pmatrix_expm(job->S, K, job->B, job->C, 0, job->d_left, bls, job->d_rates, 0,
+                                 mats, job->dtype, stream)
+               : hx_pmatrix(job->S, K, job->B, job->C, 0, job->d_lambda, job->d_left, job->d_right,
+                            bls, job->d_rates, 0, mats, job->dtype, stream);
   if (rc != HX_OK) return rc;
   if (grad) {
-    rc = hx_pmatrix_dtau(job->S, K, job->B, job->C, 0, job->d_lambda, job->d_left, job->d_right,
-                         bls, job->d_rates, 0, mats_dt, job->dtype, stream);
+    rc = job->general
+             ? hx_pmatrix_expm_dtau(job->S, K, job->B, job->C, 0, job->d_left, bls, job->d_rates, 0,
+                                    mats_dt, job->dtype, stream)
+             : hx_pmatrix_dtau(job->S, K, job->B, job->C, 0, job->d_lambda, job->d_left,
+                               job->d_right, bls, job->d_rates, 0, mats_dt, job->dtype, stream);
     if (rc != HX_OK) return rc;
   }
   double* ll_tmp = reinterpret_cast<double*>(base + w.loglik_tmp);
@@ -433,6 +468,8 @@ int hx_job_eval_model(hx_job_t job, int K, const double* bls, int eigen_batched,
   HX_REQUIRE(job && bls && lambda && left && right && rates && props && freqs && out_loglik &&
                  workspace && K >= 1,
              HX_ERR_INVALID_ARGUMENT, "null argument");
+  HX_REQUIRE(!job->general, HX_ERR_UNSUPPORTED,
+             "a job with a general rate matrix takes no per-request eigensystem");
   const hx::ModelWs w = hx::model_ws(job, K, eigen_batched);
   HX_REQUIRE(workspace_bytes >= w.total, HX_ERR_WORKSPACE_TOO_SMALL, "workspace too small");
   const hx::ModelArgs a{eigen_batched, lambda, left, right, rates, props, freqs, model_batched,

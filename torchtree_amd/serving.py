@@ -57,9 +57,12 @@ class ResidentJob:
     """One alignment resident on the device (``hx_job_create``)."""
 
     def __init__(self, name, post_indexing, codes, table, weights, q_norm, freqs, rates, props,
-                 dtype=torch.float32, slice_patterns=0, device=None, kind="dna", products="fp32"):
+                 dtype=torch.float32, slice_patterns=0, device=None, kind="dna", products="fp32",
+                 general=False):
         """``products``: for FP32 jobs, "fp32" or "bf16x3" (dense-state products as split
-        BF16 on the matrix cores, FP32 accumulation)."""
+        BF16 on the matrix cores, FP32 accumulation). ``general``: ``q_norm`` is a
+        non-reversible normalised rate matrix (``hx_job_create_general``: P = exp(Q tau) per
+        request, at most 32 states) instead of one reversible with respect to ``freqs``."""
         from .likelihood import get_plan
 
         lib = _lib.load()
@@ -76,7 +79,11 @@ class ResidentJob:
         props = np.ascontiguousarray(np.asarray(props, dtype=np.float64).reshape(-1))
         self.C = rates.shape[0]
         weights = np.ascontiguousarray(np.asarray(weights, dtype=np.float64))
-        lam, left, right = _host_eigen(torch.as_tensor(q_norm), torch.as_tensor(freqs))
+        self.general = bool(general)
+        if self.general:
+            qg = np.ascontiguousarray(torch.as_tensor(q_norm).detach().to("cpu", F64).numpy())
+        else:
+            lam, left, right = _host_eigen(torch.as_tensor(q_norm), torch.as_tensor(freqs))
         fr = np.ascontiguousarray(np.asarray(freqs, dtype=np.float64))
         self.plan = get_plan(self.T, post_indexing, device)
         h = ctypes.c_void_p()
@@ -84,14 +91,21 @@ class ResidentJob:
         prev = torch.cuda.current_device()
         torch.cuda.set_device(device)
         try:
-            rc = lib.hx_job_create(self.plan.handle, self.S, self.N, self.C, code, int(slice_patterns),
-                                   codes.ctypes.data, table.ctypes.data, table.shape[0],
-                                   weights.ctypes.data, lam.ctypes.data, left.ctypes.data,
-                                   right.ctypes.data, rates.ctypes.data, props.ctypes.data,
-                                   fr.ctypes.data, device, ctypes.byref(h))
+            if self.general:
+                rc = lib.hx_job_create_general(
+                    self.plan.handle, self.S, self.N, self.C, code, int(slice_patterns),
+                    codes.ctypes.data, table.ctypes.data, table.shape[0], weights.ctypes.data,
+                    qg.ctypes.data, rates.ctypes.data, props.ctypes.data, fr.ctypes.data, device,
+                    ctypes.byref(h))
+            else:
+                rc = lib.hx_job_create(self.plan.handle, self.S, self.N, self.C, code,
+                                       int(slice_patterns), codes.ctypes.data, table.ctypes.data,
+                                       table.shape[0], weights.ctypes.data, lam.ctypes.data,
+                                       left.ctypes.data, right.ctypes.data, rates.ctypes.data,
+                                       props.ctypes.data, fr.ctypes.data, device, ctypes.byref(h))
         finally:
             torch.cuda.set_device(prev)
-        _lib.check(rc, "hx_job_create")
+        _lib.check(rc, "hx_job_create_general" if self.general else "hx_job_create")
         self.handle = h
         self.slice = min(slice_patterns, self.N) if slice_patterns > 0 else self.N  # as hx_job_create
         self.resident_bytes = int(lib.hx_job_resident_bytes(h))
