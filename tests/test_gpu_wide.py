@@ -203,7 +203,7 @@ def test_engine_mg94(gpu, dtype, rl, rg):
 
 
 def test_mfma_lane_layout(gpu):
-    """The 16x16x4 FP32 MFMA operand/result lane layout the dense kernels assume."""
+    """The MFMA operand/result lane layouts the kernels assume (16x16x4 FP32/FP64, BF16, 4x4x1)."""
     print("case: mfma layout selftest", flush=True)
     import ctypes
 

@@ -657,14 +657,23 @@ __global__ __launch_bounds__(64 * CP) void k_pre(PassArgs<R> a, const R* __restr
 //   G_h[c][i][j] = sum over patterns of (w p_c / lik) Ut_h[c][i] L_h[c][j]
 //   U_h  = P_h^T Ut_h with one power-of-two rescale over all categories
 // (tree_likelihood.py:40-278 through autograd). Every term of G is a ratio,
-// so the scales of U and L cancel. The pattern sum of G is a 16 x 64 x 16
-// product per branch on the matrix cores (v_mfma_f32_16x16x4f32): each wave
-// writes (w p_c / lik) Ut (rows 4c + i) and L (rows 4c + j) to LDS, pattern
-// minor, and reads them back as operands; the diagonal category blocks of the
-// product are G. Adjoints waiting on the walk's stack live in registers for
+// so the scales of U and L cancel. The pattern sum of G runs on the matrix
+// cores: each wave writes Ut (rows 4c + i) and (w / lik) L (rows 4c + j) to
+// LDS, pattern minor, and reads them back as the operands of 16
+// v_mfma_f32_4x4x1_16b_f32, whose block 4c + q sums category c's 4 x 4 outer
+// products over patterns 16q .. 16q + 15; a row rotation then adds the four
+// blocks q of each category. Adjoints waiting on the walk's stack live in registers for
 // depths below RS and in the pushed child's own partials slot (in place, its
 // post-order partial has been read) deeper down.
 constexpr int kXS = 68;  // row stride (floats) of the LDS pattern transposes
+
+// v + (v from the lane `ROT` below in the same row of 16 lanes, cyclically): DPP row_ror
+template <int ROT>
+__device__ __forceinline__ float add_row_ror(float v) {
+  static_assert(ROT > 0 && ROT < 16, "row rotation");
+  return v + __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v),
+                                                                 0x120 + ROT, 0xF, 0xF, true));
+}
 
 template <int CM>
 struct PreIn {
@@ -679,9 +688,10 @@ template <int TIPS, int CM, bool EXACT, int RS, bool TAU>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
     !EXACT ? 1 : (RS == 0 ? 4 : 3)))) void k_pre_walk(PassArgs<float> a, const float* __restrict__ mats,
                                                   const float* __restrict__ mats_dt,
-                                                  const int* __restrict__ prog, int nprog, int W) {
+                                                  const int* __restrict__ prog, int nprog) {
   using R = float;
   using f4 = float __attribute__((ext_vector_type(4)));
+  constexpr int W = kPreWalkW;  // waves (tiles) per workgroup
   using H0 = std::integral_constant<int, 0>;
   using H1 = std::integral_constant<int, 1>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -793,7 +803,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
   for (int c = 0; c < CM; ++c) U[c] = s0[c] = s1[c] = zero4<R>();
   float* xs = xt;              // rows 4c + i: (w p_c / lik) Ut[c][i]
   float* ls = xt + 16 * kXS;    // rows 4c + j: L[c][j] (a tip: rows j)
-  const int r16 = lane & 15, g4 = lane >> 4;
+  const int g4 = lane >> 4;
 
   // One entry per trip. Part 1 consumes the prefetched data (the adjoint, m,
   // the statistic's transposes, a pushed adjoint's store); then the next
@@ -947,23 +957,40 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
         for (int s = 0; s < 4; ++s) ls[s * kXS + lane] = coef * tv[h].x[s];
       }
     };
+    // Block b = 4c + q of the 4x4x1 product: lane 4b + i gives A[b][i] = Ut[c][i] and lane
+    // 4b + j gives B[b][j] = (w / lik) L[c][j] of pattern 16q + m in product m; register i of
+    // lane 4b + j accumulates G[c][i][j] over those patterns.
+    const int srow = 4 * g4 + (lane & 3), scol = 16 * ((lane >> 2) & 3);
     auto stat_mfma = [&](int h) {
       __builtin_amdgcn_wave_barrier();
-      const int rl = ((tips >> h) & 1) ? (r16 & 3) : r16;
+      const int rl = ((tips >> h) & 1) ? (lane & 3) : srow;
       f4 g = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const f4 A = *reinterpret_cast<const f4*>(xs + r16 * kXS + 16 * g4 + 4 * u);
-        const f4 Bv = *reinterpret_cast<const f4*>(ls + rl * kXS + 16 * g4 + 4 * u);
+        const f4 A = *reinterpret_cast<const f4*>(xs + srow * kXS + scol + 4 * u);
+        const f4 Bv = *reinterpret_cast<const f4*>(ls + rl * kXS + scol + 4 * u);
 #pragma unroll
-        for (int e = 0; e < 4; ++e) g = __builtin_amdgcn_mfma_f32_16x16x4f32(A[e], Bv[e], g, 0, 0, 0);
+        for (int e = 0; e < 4; ++e) g = __builtin_amdgcn_mfma_f32_4x4x1f32(A[e], Bv[e], g, 0, 0, 0);
       }
       __builtin_amdgcn_wave_barrier();
       return g;
     };
-    f4 gacc[2];
+    // The four pattern groups of each category (lanes 16c + 4q + j) are added by row
+    // rotations; lane 16c + j then holds G_h[c][i][j] in register i and writes it to the
+    // wave's slot of the block's sum, double-buffered by the entry's parity (the other
+    // buffer's readers have all passed the previous entry's barrier).
+    const int par = i & 1;
+    auto stat_put = [&](int h, f4 g) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) g[r] = add_row_ror<8>(add_row_ror<4>(g[r]));
+      if (((lane >> 2) & 3) == 0 && g4 < C) {
+        float* dst = red + ((par * 2 + h) * W + wave) * 64 + g4 * 16 + (lane & 3);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) dst[4 * r] = g[r];
+      }
+    };
     stat_rows(H0{});
-    gacc[0] = stat_mfma(0);
+    stat_put(0, stat_mfma(0));
     stat_rows(H1{});
     // adjoints of internal children, computed straight into their registers
     // (the child index is a compile-time constant in each branch)
@@ -1017,17 +1044,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
     fetch(f0, f1, f2, cur);
 
     // ---------------------------------------------------------------- part 2
-    gacc[1] = stat_mfma(1);
-    // lane 20 c + j holds G_h[c][i][j] in register i
-    const int par = i & 1;
-    if ((lane & 15) >> 2 == g4 && g4 < C) {
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        float* dst = red + ((par * 2 + h) * W + wave) * 64 + g4 * 16 + (lane & 3);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) dst[4 * q] = gacc[h][q];
-      }
-    }
+    stat_put(1, stat_mfma(1));
     lds_barrier();
     {
       // thread t sums its entry over the W tiles; threads with the same entry
@@ -1051,7 +1068,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
 template <int CM, bool EXACT>
 static void pre_walk_kernel(int tips, bool tau,
                             void (*&kern)(PassArgs<float>, const float*, const float*, const int*,
-                                          int, int)) {
+                                          int)) {
   if (tau)
     kern = tips == HX_TIPS_CODES ? k_pre_walk<HX_TIPS_CODES, CM, EXACT, (CM == 4 ? 1 : 2), true>
                                  : k_pre_walk<HX_TIPS_DENSE, CM, EXACT, 2, true>;
@@ -1200,7 +1217,7 @@ int launch_pre_walk(const PassArgs<float>& a, const int32_t* prog, int nprog, in
   const size_t lds = (size_t)W * 2 * 16 * kXS * sizeof(float) + (size_t)4 * W * 64 * sizeof(float) +
                      tbl_bytes(tips, a.ncodes, sizeof(float));
   HX_REQUIRE(lds <= kDnaWalkLdsMax, HX_ERR_UNSUPPORTED, "pre-order walk: LDS");
-  void (*kern)(PassArgs<float>, const float*, const float*, const int*, int, int) = nullptr;
+  void (*kern)(PassArgs<float>, const float*, const float*, const int*, int) = nullptr;
   const bool tau = a.mats_dt != nullptr;
   switch (a.C) {
     case 1: pre_walk_kernel<1, true>(tips, tau, kern); break;
@@ -1211,7 +1228,7 @@ int launch_pre_walk(const PassArgs<float>& a, const int32_t* prog, int nprog, in
   if (lds > 64 * 1024)
     HX_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  kern<<<dim3(a.nchunks, a.K), 64 * W, lds, st>>>(a, a.mats, a.mats_dt, prog, nprog, W);
+  kern<<<dim3(a.nchunks, a.K), 64 * W, lds, st>>>(a, a.mats, a.mats_dt, prog, nprog);
   HX_HIP_TRY(hipGetLastError());
   return HX_OK;
 }

@@ -1345,6 +1345,40 @@ __global__ void k_mfma_layout_bf16(float* out) {
   }
 }
 
+// The 16-block 4x4x1 FP32 product of the four-state pre-order walk's branch statistic:
+// A[b][i] in lane 4b + i, B[b][j] in lane 4b + j, D[b][i][j] in register i of lane 4b + j.
+// A[b][i] = 1 + i + 4b and B[b][j] = 1 + 0.5 j - 0.25 b (products exact in FP32).
+__global__ void k_mfma_layout_4x4(float* out) {
+  const int lane = threadIdx.x;
+  const int b = lane >> 2, x = lane & 3;
+  const float av = 1.f + (float)x + 4.f * (float)b;
+  const float bv = 1.f + 0.5f * (float)x - 0.25f * (float)b;
+  const vec4_t<float> d = __builtin_amdgcn_mfma_f32_4x4x1f32(av, bv, vec4_t<float>{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) out[(b * 4 + r) * 4 + x] = d[r];
+}
+
+static int mfma_layout_check_4x4(hipStream_t st, int* bad) {
+  float* d = nullptr;
+  HX_HIP_TRY(hipMalloc(&d, 256 * sizeof(float)));
+  k_mfma_layout_4x4<<<1, 64, 0, st>>>(d);
+  float host[256];
+  hipError_t e = hipMemcpyAsync(host, d, sizeof(host), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  hipFree(d);
+  if (e != hipSuccess) {
+    set_error(std::string("mfma 4x4 self-test: ") + hipGetErrorString(e));
+    return HX_ERR_HIP;
+  }
+  for (int b = 0; b < 16; ++b)
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) {
+        const double ref = (1.0 + i + 4.0 * b) * (1.0 + 0.5 * j - 0.25 * b);
+        if ((double)host[(b * 4 + i) * 4 + j] != ref) ++*bad;
+      }
+  return HX_OK;
+}
+
 template <int N>
 static int mfma_layout_check_bf16(hipStream_t st, int* bad) {
   float* d = nullptr;
@@ -1404,6 +1438,7 @@ extern "C" int hx_selftest_mfma_layout(int* mismatches, hx_stream_t stream) {
   if (rc == HX_OK) rc = hx::mfma_layout_check<double>(st, &bad);
   if (rc == HX_OK) rc = hx::mfma_layout_check_bf16<8>(st, &bad);
   if (rc == HX_OK) rc = hx::mfma_layout_check_bf16<4>(st, &bad);
+  if (rc == HX_OK) rc = hx::mfma_layout_check_4x4(st, &bad);
   *mismatches = bad;
   return rc;
 }
