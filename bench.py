@@ -111,7 +111,7 @@ def walk_drops(T, post, dev, S, C, N, K, rb):
     d0, d1 = ctypes.c_int(), ctypes.c_int()
     _lib.check(_lib.load().hx_plan_walk_info(plan.handle, None, None, None, ctypes.byref(d0),
                                              ctypes.byref(d1)))
-    return d1.value if C in (1, 2, 4) else d0.value
+    return d0.value
 
 
 def algorithmic_bytes(T, N, C, K, S, rb, nchunks, drops=0):

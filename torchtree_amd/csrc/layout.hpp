@@ -159,12 +159,12 @@ inline size_t workspace_bytes_for(int taxa, int S, int N, int C, int K, int dtyp
 // Register mode of the post-order walk's drop bits (walk.hpp) for a call, or -1 when every
 // internal partial is stored: cherries are dropped for four-state FP32 calls with tip codes
 // whose gradient runs as the pre-order walk (use_pre_walk), the only reader that rebuilds
-// them. Mode 1 where the post-order walk keeps stack depths in registers (FP32 at 1, 2 or 4
-// categories), else mode 0.
+// them. Mode 0: the post-order walk keeps no stack depths in registers (mode 1 is the
+// program's mode for a walk that keeps depths 2..4 there).
 inline int walk_drop_mode(int S, int C, int N, int K, int dtype, int tip_kind) {
   if (S != 4 || dtype == HX_F64 || tip_kind != HX_TIPS_CODES || !use_pre_walk(S, C, 4, N, K))
     return -1;
-  return (C == 1 || C == 2 || C == 4) ? 1 : 0;
+  return 0;
 }
 
 // Workspace of one resident-job evaluation (hx_job_eval): P (and dP/dtau), one pattern
