@@ -92,8 +92,7 @@ def walk_need(post, T):
 
 def pre_walk_used(S, C, rb, N, K):
     """The four-state FP32 pre-order runs as one walk (mirrors use_pre_walk, csrc/layout.hpp)."""
-    groups = ((N + 63) // 64 + 3) // 4 * K
-    return S == 4 and rb == 4 and C <= 4 and groups >= 256
+    return S == 4 and rb == 4 and C <= 4
 
 
 def walk_drops(T, post, dev, S, C, N, K, rb):

@@ -158,9 +158,8 @@ def test_engine_grad_model(gpu, dtype, rtol_l, rtol_g):
         assert err < rtol_g, (name, err)
 
 
-# (24, 33000, ., .) with K = 2 has ceil(516 / 4) * 2 = 258 workgroups: the FP32
-# pre-order runs as the walk (C = 1, 3, 4 and dense tips); the smaller shapes
-# run it by level
+# The FP32 pre-order runs as the walk at C = 1, 3 and 4 (dense tips included), by level at
+# C = 8 and 16; at (24, 33000) it has ceil(516 / 4) * 2 = 258 workgroups
 @pytest.mark.parametrize("T,N,C,dense", [(70, 700, 8, False), (40, 260, 16, False), (30, 333, 4, True),
                                          (24, 33000, 4, False), (24, 33000, 1, False),
                                          (24, 33000, 3, False), (24, 33000, 4, True)])
@@ -261,3 +260,34 @@ def test_pre_walk_tree_shapes(gpu, shape):
     for a, b in ((m2.grad, mats.grad), (f2.grad, freqs.grad), (p2.grad, pr.grad)):
         err = float((a.double().cpu() - b).norm() / b.norm())
         assert err < 1e-5, err
+
+
+@pytest.mark.parametrize("C", [1, 4])
+def test_fp32_gradient_batch_invariant(gpu, C):
+    """An FP32 four-state sample's log-likelihood and d logL/dP are bitwise the same evaluated
+    alone (K = 1: 8 workgroups) and in a batch of 32 (256 workgroups): every batch size runs
+    the same pre-order walk and chunking (the reference asserts batch == single,
+    test/test_tree_likelihood.py:263-265)."""
+    _flush(f"case: FP32 batch == single C={C}")
+    from torchtree_amd import likelihood as hl
+    from torchtree_amd import synthetic
+
+    K = 32
+    wl = synthetic.make_workload(24, 2048, K, categories=C, model="gtr", seed=11, bl_mean=0.1)
+    post = [tuple(int(x) for x in r) for r in wl.topo.postorder]
+    tips = wl.tip_partials() + [None] * (wl.T - 1)
+    q = omodels.gtr_q(wl.exch, wl.freqs)
+    rates = torch.linspace(0.3, 2.0, C, dtype=F64)
+    tau = wl.bls.unsqueeze(-1) * rates.reshape(1, 1, -1)
+    mats = omodels.p_t_eigh(q, wl.freqs, tau).to(gpu, torch.float32)
+    freqs = wl.freqs.reshape(1, -1).to(gpu, torch.float32)
+    props = torch.full((C, 1, 1), 1.0 / C, dtype=torch.float32, device=gpu)
+    mb = mats.clone().requires_grad_(True)
+    lb = hl.calculate_treelikelihood_discrete(tips, wl.weights, post, mb, freqs, props)
+    lb.sum().backward()
+    for j in (0, 17, K - 1):
+        mj = mats[j:j + 1].clone().requires_grad_(True)
+        lj = hl.calculate_treelikelihood_discrete(tips, wl.weights, post, mj, freqs, props)
+        lj.sum().backward()
+        assert torch.equal(lj[0], lb[j]), j
+        assert torch.equal(mj.grad[0], mb.grad[j]), j

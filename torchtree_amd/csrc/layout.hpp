@@ -61,26 +61,23 @@ inline int wide_nt(int C, int S, int rb) {
 // S == 4 uses tiles of 64 patterns (a wave per category, a lane per pattern);
 // other S use tiles of wide_nt() patterns handled by (child, category, state)
 // lanes.
-// The chunking depends on N (and the tile shape) only, never on K, except
-// for the choice of the four-state FP32 pre-order walk (use_pre_walk): every
-// other sample's sums run in the same order whatever batch it is evaluated
-// in, so its result is bitwise identical alone and in a batch.
+// The chunking and the kernels depend on N (and the tile shape) only, never on K: a sample's
+// sums run in the same order whatever batch it is evaluated in, so its result is bitwise
+// identical alone and in a batch.
 constexpr int kTargetChunks = 128;
 
 // The four-state FP32 pre-order walk with at most kPreWalkMaxC categories
 // sums its branch statistics over the kPreWalkW tiles of a workgroup: one
-// chunk per kPreWalkW tiles. It walks the tree's entries one after another,
-// so it is used when it has at least kPreWalkMinGroups workgroups
-// (ceil(tiles / 4) x K) to fill the chip; smaller calls run the pre-order
-// by level, which is parallel over each level's nodes (measured crossover).
-// The choice depends on K, so an FP32 four-state gradient may differ in its
-// last bits between batch sizes on the two sides of the threshold.
+// chunk per kPreWalkW tiles. It serves every such call, also those with too
+// few workgroups (ceil(tiles / 4) x K) to fill the chip, where a pre-order by
+// level would be faster: choosing by size would make a sample's FP32 gradient
+// depend on the batch it is evaluated in.
 constexpr int kPreWalkW = 4;
 constexpr int kPreWalkMaxC = 4;
-constexpr long kPreWalkMinGroups = 256;
 inline bool use_pre_walk(int S, int C, int rb, int N, int K) {
-  const long groups = (long)((N + 63) / 64 + kPreWalkW - 1) / kPreWalkW * K;
-  return S == 4 && rb == 4 && C <= kPreWalkMaxC && groups >= kPreWalkMinGroups;
+  (void)N;
+  (void)K;
+  return S == 4 && rb == 4 && C <= kPreWalkMaxC;
 }
 
 inline Tiling make_tiling(int N, int C, int K, int S, int rb) {

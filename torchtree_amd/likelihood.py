@@ -244,9 +244,8 @@ class Pruning:
         """Kernel launches of one pass (the walks are one launch for the whole tree)."""
         if S == 4 and which == "post":
             return 2 + int(self.tips.kind == _lib.HX_TIPS_CODES)  # pad codes, walk, root sums
-        groups = ((self.N + 63) // 64 + 3) // 4 * K
-        if S == 4 and which == "pre" and code != _lib.HX_F64 and C <= 4 and groups >= 256:
-            return 2  # walk, statistic reduction (mirrors use_pre_walk, csrc/common.hpp)
+        if S == 4 and which == "pre" and code != _lib.HX_F64 and C <= 4:
+            return 2  # walk, statistic reduction (mirrors use_pre_walk, csrc/layout.hpp)
         return (self.plan.post_levels if which == "post" else self.plan.pre_levels) + 1
 
     def forward(self, mats, freqs, props, want_root_grads=True):
