@@ -7,7 +7,7 @@
   the divided differences.
 * c2: GTR + Weibull4, 512 taxa, K = 32 on a 2,048-pattern slice (256 workgroups of the FP32
   pre-order walk); the oracle evaluates 4 of the 32 samples.
-* c3: GTR + Weibull4, 1,024 taxa, K = 16 on a 4,096-pattern slice (walk size), in the reference
+* c3: GTR + Weibull4, 1,024 taxa, K = 16 on a 4,096-pattern slice, in the reference
   harness's unconstrained coordinates (``benchmarks/benchmark.py:919-936``: 5 stick-breaking
   rate coordinates and 3 frequency coordinates); the oracle evaluates 2 of the samples, and FP64
   central differences (step 1e-6) check 20 branch lengths and all 8 model coordinates.
