@@ -291,3 +291,43 @@ def test_fp32_gradient_batch_invariant(gpu, C):
         lj.sum().backward()
         assert torch.equal(lj[0], lb[j]), j
         assert torch.equal(mj.grad[0], mb.grad[j]), j
+
+
+@pytest.mark.parametrize("T,N,K,C", [(2, 1, 1, 1), (2, 65, 3, 2), (3, 1, 2, 3), (3, 200, 1, 4),
+                                     (5, 64, 3, 4)])
+@pytest.mark.parametrize("dtype,rtol_l,rtol_g", [(F64, 1e-12, 1e-10), (torch.float32, 1e-6, 1e-5)])
+def test_smallest_trees(gpu, T, N, K, C, dtype, rtol_l, rtol_g):
+    """Two and three taxa, one pattern, ragged tiles, odd category counts: the whole-tree walks
+    with a single entry, a workgroup with one real tile of four, and the padding lanes."""
+    _flush(f"case: smallest trees T={T} N={N} K={K} C={C} {dtype}")
+    from torchtree_amd import likelihood as hl
+
+    rng = np.random.default_rng(100 * T + N)
+    post = [(2, 0, 1)] if T == 2 else ([(3, 0, 1), (4, 3, 2)] if T == 3 else
+                                       [(5, 0, 1), (6, 2, 3), (7, 5, 6), (8, 7, 4)])
+    B = 2 * T - 2
+    codes = rng.integers(0, 5, size=(T, N))  # 4 = gap (all ones)
+    table = np.vstack([np.eye(4), np.ones((1, 4))])
+    tips = [torch.tensor(table[codes[t]].T.copy(), dtype=F64) for t in range(T)]
+    weights = torch.tensor(rng.integers(1, 4, size=N), dtype=F64)
+    exch = torch.tensor(rng.gamma(2.0, 2.0, size=6), dtype=F64)
+    freqs = torch.tensor(rng.dirichlet(np.full(4, 10.0)), dtype=F64)
+    bls = torch.tensor(rng.exponential(0.2, size=(K, B)), dtype=F64)
+    rates = torch.linspace(0.4, 1.8, C, dtype=F64)
+    q = omodels.gtr_q(exch, freqs)
+    mats = omodels.p_t_eigh(q, freqs, bls.unsqueeze(-1) * rates.reshape(1, 1, -1)).detach()
+    mats.requires_grad_(True)
+    fr = freqs.reshape(1, -1).clone().requires_grad_(True)
+    pr = torch.full((C, 1, 1), 1.0 / C, dtype=F64, requires_grad=True)
+    ref = opruning.treelik_discrete_rescaled(tips + [None] * (T - 1), weights, post, mats, fr, pr)
+    ref.sum().backward()
+    m2 = mats.detach().to(gpu, dtype).requires_grad_(True)
+    f2 = fr.detach().to(gpu, dtype).requires_grad_(True)
+    p2 = pr.detach().to(gpu, dtype).requires_grad_(True)
+    got = hl.calculate_treelikelihood_discrete(tips + [None] * (T - 1), weights, post, m2, f2, p2)
+    rel = (got.double().cpu() - ref.detach()).abs() / ref.detach().abs()
+    assert float(rel.max()) < rtol_l
+    got.sum().backward()
+    for a, b in ((m2.grad, mats.grad), (f2.grad, fr.grad), (p2.grad, pr.grad)):
+        err = float((a.double().cpu() - b).norm() / b.norm())
+        assert err < rtol_g, err
