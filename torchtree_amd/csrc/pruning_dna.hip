@@ -1113,8 +1113,9 @@ int launch_post_dna(const PassArgs<R>& a, const int32_t* prog, const int32_t* se
              "post-order walk: tip codes of the tile exceed LDS (too many taxa)");
   HX_REQUIRE(sizeof(R) == 8 || (size_t)a.C * a.np * 4 * sizeof(R) < ((size_t)1 << 31),
              HX_ERR_UNSUPPORTED, "post-order walk: a node's rows of one sample exceed 2 GiB");
-  HX_REQUIRE(a.drop_mode < 0 || (sizeof(R) == 4 && tips == HX_TIPS_CODES &&
-                                 (a.drop_mode == 0 || a.C == 1 || a.C == 2 || a.C == 4)),
+  // mode 1 would drop cherries that a walk with register depths takes from registers; these
+  // launches keep none (NR below), so only mode 0 matches them
+  HX_REQUIRE(a.drop_mode < 0 || (sizeof(R) == 4 && tips == HX_TIPS_CODES && a.drop_mode == 0),
              HX_ERR_UNSUPPORTED, "post-order walk: drop mode without the register depths it assumes");
   if (tips == HX_TIPS_CODES) {
     const size_t npad = (size_t)a.ntiles * 64, total = (size_t)a.T * npad;
