@@ -198,14 +198,15 @@ __device__ __forceinline__ void acc_outer(R (&g)[16], R w, const V4<R>& u, const
 // node's stores so the wait for it does not wait for them, and its exponent
 // comes from the lane's LDS stack. grid = (ntiles, ceil(K / W)), block = 64 W.
 // CM: categories rounded up to a power of two (EXACT: C == CM).
-// NRS > 0: results at stack depths kRegDepth0 .. kRegDepth0 + NRS - 1 also
-// stay in registers, so their consumers do not read them back from memory.
+// No stack depth stays in registers: keeping depths 2..4 there (the program's
+// register mode 1, walk.hpp) takes the four-category FP32 walk from 69 to 114
+// VGPRs, 4 waves per SIMD instead of 6 (the LDS limit), and the walk is
+// latency-bound, so the extra waves gain more than the read-backs cost
+// (measured at 512 taxa, 100,000 patterns, K = 32: 21.2 -> 20.0 ms).
 // a.drop_mode >= 0: the cherries whose drop bit (walk.hpp) is set are not
 // stored; their store instruction is still issued, as a raw buffer store
 // with an empty range (the hardware discards it), so every entry issues the
 // same vector memory operations and the waits stay exact.
-constexpr int kRegDepth0 = kWalkRegDepth0;
-
 __device__ __forceinline__ void store_row_f32(const __amdgpu_buffer_rsrc_t& r, unsigned off,
                                               const V4<float>& v) {
   typedef uint32_t u4 __attribute__((ext_vector_type(4)));
@@ -214,11 +215,10 @@ __device__ __forceinline__ void store_row_f32(const __amdgpu_buffer_rsrc_t& r, u
   __builtin_amdgcn_raw_buffer_store_b128(w, r, (int)off, 0, 0);
 }
 
-template <typename R, int TIPS, int CM, bool EXACT, int NRS>
+template <typename R, int TIPS, int CM, bool EXACT>
 __global__ __launch_bounds__(CM >= 8 ? 256 : 1024) void k_post_walk(PassArgs<R> a, const R* __restrict__ mats,
                                                     const int4* __restrict__ prog, int nprog,
                                                     int ne, int W) {
-  static_assert(NRS == 0 || NRS == kWalkRegDepths, "register depths of the drop bits");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int C = EXACT ? CM : a.C;
   const int wave = uni(threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -266,17 +266,11 @@ __global__ __launch_bounds__(CM >= 8 ? 256 : 1024) void k_post_walk(PassArgs<R> 
   // unused): every iteration then issues the same vector memory operations,
   // so the wait for these loads counts only the older ones and not this
   // node's stores, and the placeholder loads never reach HBM.
-  auto in_regs = [&](int slot) { return slot >= kRegDepth0 && slot < kRegDepth0 + NRS; };
   auto stacked_of = [&](const int4& f0, const int4& f1) {
-    if ((f1.z & 0xff) == 2 && !in_regs((f0.w & 0xff) - 1)) return f0.y;
-    if (((f1.z >> 8) & 0xff) == 2 && !in_regs(((f0.w >> 8) & 0xff) - 1)) return f0.z;
+    if ((f1.z & 0xff) == 2) return f0.y;
+    if (((f1.z >> 8) & 0xff) == 2) return f0.z;
     return -1;
   };
-  // register copies of the results at depths kRegDepth0 + 0, 1, 2
-  static_assert(NRS >= 0 && NRS <= 3, "at most three register depths");
-  V4<R> g0[CM], g1[CM], g2[CM];
-#pragma unroll
-  for (int c = 0; c < CM; ++c) g0[c] = g1[c] = g2[c] = zero4<R>();
   const R* hot = mats + (size_t)k * a.B * C * 16 + (lane & 3) * 4;
   auto readback = [&](int ch, V4<R>(&x)[CM]) {
     const bool real = ch >= 0;
@@ -311,18 +305,8 @@ __global__ __launch_bounds__(CM >= 8 ? 256 : 1024) void k_post_walk(PassArgs<R> 
       for (int c = 0; c < CM; ++c) xl[c] = y[c];
       El = E;
     } else {
-      if (in_regs(sl)) {
 #pragma unroll
-        for (int c = 0; c < CM; ++c)
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const R u = sl == kRegDepth0 + 1 ? g1[c].x[q] : g2[c].x[q];
-            xl[c].x[q] = sl == kRegDepth0 ? g0[c].x[q] : u;
-          }
-      } else {
-#pragma unroll
-        for (int c = 0; c < CM; ++c) xl[c] = xs[c];
-      }
+      for (int c = 0; c < CM; ++c) xl[c] = xs[c];
       El = est[sl * 64 + lane];
     }
     if (kr == 0) {
@@ -334,18 +318,8 @@ __global__ __launch_bounds__(CM >= 8 ? 256 : 1024) void k_post_walk(PassArgs<R> 
       for (int c = 0; c < CM; ++c) xr[c] = y[c];
       Er = E;
     } else {
-      if (in_regs(sr)) {
 #pragma unroll
-        for (int c = 0; c < CM; ++c)
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const R u = sr == kRegDepth0 + 1 ? g1[c].x[q] : g2[c].x[q];
-            xr[c].x[q] = sr == kRegDepth0 ? g0[c].x[q] : u;
-          }
-      } else {
-#pragma unroll
-        for (int c = 0; c < CM; ++c) xr[c] = xs[c];
-      }
+      for (int c = 0; c < CM; ++c) xr[c] = xs[c];
       Er = est[sr * 64 + lane];
     }
     // P of (l, c) and (r, c); the loads of category c + 1 are issued before
@@ -394,19 +368,6 @@ __global__ __launch_bounds__(CM >= 8 ? 256 : 1024) void k_post_walk(PassArgs<R> 
         if (EXACT || c < C) st4(p + (size_t)c * cstride, y[c]);
     }
     est[so * 64 + lane] = E;
-    // only a result that will wait on the stack (the next entry does not
-    // consume it) needs its register copy
-    const bool waits = (f1.z & 0xff) != 1 && ((f1.z >> 8) & 0xff) != 1;
-    if (NRS > 0 && waits && in_regs(so)) {
-#pragma unroll
-      for (int c = 0; c < CM; ++c)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          g0[c].x[q] = so == kRegDepth0 ? y[c].x[q] : g0[c].x[q];
-          if (NRS > 1) g1[c].x[q] = so == kRegDepth0 + 1 ? y[c].x[q] : g1[c].x[q];
-          if (NRS > 2) g2[c].x[q] = so == kRegDepth0 + 2 ? y[c].x[q] : g2[c].x[q];
-        }
-    }
     e0 = f0;
     e1 = f1;
   }
@@ -1113,8 +1074,8 @@ int launch_post_dna(const PassArgs<R>& a, const int32_t* prog, const int32_t* se
              "post-order walk: tip codes of the tile exceed LDS (too many taxa)");
   HX_REQUIRE(sizeof(R) == 8 || (size_t)a.C * a.np * 4 * sizeof(R) < ((size_t)1 << 31),
              HX_ERR_UNSUPPORTED, "post-order walk: a node's rows of one sample exceed 2 GiB");
-  // mode 1 would drop cherries that a walk with register depths takes from registers; these
-  // launches keep none (NR below), so only mode 0 matches them
+  // mode 1 would drop cherries that a walk with register depths takes from registers; this
+  // walk keeps none, so only mode 0 matches it
   HX_REQUIRE(a.drop_mode < 0 || (sizeof(R) == 4 && tips == HX_TIPS_CODES && a.drop_mode == 0),
              HX_ERR_UNSUPPORTED, "post-order walk: drop mode without the register depths it assumes");
   if (tips == HX_TIPS_CODES) {
@@ -1124,16 +1085,10 @@ int launch_post_dna(const PassArgs<R>& a, const int32_t* prog, const int32_t* se
   }
   dim3 grid(a.ntiles, (a.K + W - 1) / W);
   const int4* pg = reinterpret_cast<const int4*>(prog);
-  // No register depths (NRS = 0): keeping depths 2..4 in registers takes the four-category
-  // FP32 walk from 69 to 114 VGPRs, 4 waves per SIMD instead of 6 (the LDS limit), and the
-  // walk is latency-bound, so the extra waves gain more than the read-backs they cause cost
-  // (measured at 512 taxa, 100,000 patterns, K = 32: 21.2 -> 20.0 ms).
-  constexpr int NR = 0;
 #define HX_WALK(CMV, EX)                                                                     \
   do {                                                                                       \
-    constexpr int NRV = (CMV <= 4 && EX) ? NR : 0;                                          \
-    auto kern = tips == HX_TIPS_CODES ? k_post_walk<R, HX_TIPS_CODES, CMV, EX, NRV>          \
-                                      : k_post_walk<R, HX_TIPS_DENSE, CMV, EX, NRV>;         \
+    auto kern = tips == HX_TIPS_CODES ? k_post_walk<R, HX_TIPS_CODES, CMV, EX>               \
+                                      : k_post_walk<R, HX_TIPS_DENSE, CMV, EX>;              \
     if (lds > 64 * 1024)                                                                     \
       HX_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),                    \
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
