@@ -331,3 +331,40 @@ def test_smallest_trees(gpu, T, N, K, C, dtype, rtol_l, rtol_g):
     for a, b in ((m2.grad, mats.grad), (f2.grad, fr.grad), (p2.grad, pr.grad)):
         err = float((a.double().cpu() - b).norm() / b.norm())
         assert err < rtol_g, err
+
+
+@pytest.mark.parametrize("dtype,rtol_l,rtol_g", [(F64, 1e-12, 1e-10), (torch.float32, 1e-6, 1e-5)])
+def test_walks_with_a_large_code_table(gpu, dtype, rtol_l, rtol_g):
+    """About 200 distinct tip columns (random ambiguity vectors) stay on the tip-code path: the
+    walks stage a 200-row code table in LDS ahead of their stacks and transposes."""
+    _flush(f"case: large code table {dtype}")
+    from torchtree_amd import likelihood as hl
+    from torchtree_amd import synthetic
+
+    rng = np.random.default_rng(21)
+    T, N, K, C = 40, 3000, 2, 4
+    wl = synthetic.make_workload(T, N, K, categories=C, model="gtr", seed=21, bl_mean=0.1)
+    post = [tuple(int(x) for x in r) for r in wl.topo.postorder]
+    cols = np.vstack([np.eye(4), rng.uniform(0.05, 1.0, size=(196, 4))])  # 200 distinct columns
+    codes = rng.integers(0, cols.shape[0], size=(T, wl.N))
+    tips = [torch.tensor(cols[codes[t]].T.copy(), dtype=F64) for t in range(T)]
+    td = hl.TipData.from_partials(tips, "cpu")
+    assert td.kind == hl.HX_TIPS_CODES and 150 <= td.ncodes <= 256
+    rates = torch.linspace(0.3, 2.2, C, dtype=F64)
+    q = omodels.gtr_q(wl.exch, wl.freqs)
+    mats = omodels.p_t_eigh(q, wl.freqs, wl.bls.unsqueeze(-1) * rates.reshape(1, 1, -1)).detach()
+    mats.requires_grad_(True)
+    fr = wl.freqs.reshape(1, -1).clone().requires_grad_(True)
+    pr = torch.full((C, 1, 1), 1.0 / C, dtype=F64, requires_grad=True)
+    ref = opruning.treelik_discrete_rescaled(tips + [None] * (T - 1), wl.weights, post, mats, fr, pr)
+    ref.sum().backward()
+    m2 = mats.detach().to(gpu, dtype).requires_grad_(True)
+    f2 = fr.detach().to(gpu, dtype).requires_grad_(True)
+    p2 = pr.detach().to(gpu, dtype).requires_grad_(True)
+    got = hl.calculate_treelikelihood_discrete(tips + [None] * (T - 1), wl.weights, post, m2, f2, p2)
+    rel = (got.double().cpu() - ref.detach()).abs() / ref.detach().abs()
+    assert float(rel.max()) < rtol_l
+    got.sum().backward()
+    for a, b in ((m2.grad, mats.grad), (f2.grad, fr.grad), (p2.grad, pr.grad)):
+        err = float((a.double().cpu() - b).norm() / b.norm())
+        assert err < rtol_g, err
