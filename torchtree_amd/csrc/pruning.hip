@@ -82,8 +82,17 @@ __global__ void k_reduce_g(const R* __restrict__ gpart, double* __restrict__ dma
   const size_t kb = idx / CSS;
   const int b = (int)(kb % B), k = (int)(kb / B);
   const R* src = gpart + (((size_t)b * K + k) * nchunks) * CSS + q;
+  // eight loads in flight, then the same sequential sum
   double t = 0.0;
-  for (int ch = 0; ch < nchunks; ++ch) t += (double)src[(size_t)ch * CSS];
+  int ch = 0;
+  for (; ch + 8 <= nchunks; ch += 8) {
+    R v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = src[(size_t)(ch + u) * CSS];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) t += (double)v[u];
+  }
+  for (; ch < nchunks; ++ch) t += (double)src[(size_t)ch * CSS];
   dmats[idx] = t;
 }
 
@@ -96,7 +105,15 @@ __global__ void k_reduce_tau(const double* __restrict__ gtau, double* __restrict
   const int b = (int)(kb % B), k = (int)(kb / B);
   const double* src = gtau + (((size_t)b * K + k) * nchunks) * C + c;
   double t = 0.0;
-  for (int ch = 0; ch < nchunks; ++ch) t += src[(size_t)ch * C];
+  int ch = 0;
+  for (; ch + 8 <= nchunks; ch += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = src[(size_t)(ch + u) * C];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) t += v[u];
+  }
+  for (; ch < nchunks; ++ch) t += src[(size_t)ch * C];
   dtau[idx] = t;
 }
 
