@@ -1045,12 +1045,19 @@ static size_t tbl_bytes(int tips, int ncodes, size_t rb) {
 
 // Copies tip codes [T][N] into rows of ntiles * 64 bytes (zero padded) so the
 // walk stages a tile's codes with aligned 16-byte loads.
+// grid (ceil(npad / 1024), T): thread j4 writes bytes 4 j4 .. 4 j4 + 3 of row blockIdx.y as
+// one 32-bit store (npad is a multiple of 64, so the stores are aligned)
 __global__ void k_pad_codes(const uint8_t* __restrict__ codes, uint8_t* __restrict__ out, int T,
                             int N, int npad) {
-  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (size_t)T * npad) return;
-  const int t = (int)(idx / npad), j = (int)(idx % npad);
-  out[idx] = j < N ? codes[(size_t)t * N + j] : 0;
+  const int t = blockIdx.y;
+  const int j = 4 * (blockIdx.x * blockDim.x + threadIdx.x);
+  if (j >= npad) return;
+  const uint8_t* src = codes + (size_t)t * N;
+  uint32_t v = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    if (j + q < N) v |= (uint32_t)src[j + q] << (8 * q);
+  *reinterpret_cast<uint32_t*>(out + (size_t)t * npad + j) = v;
 }
 
 template <typename R>
@@ -1079,9 +1086,10 @@ int launch_post_dna(const PassArgs<R>& a, const int32_t* prog, const int32_t* se
   HX_REQUIRE(a.drop_mode < 0 || (sizeof(R) == 4 && tips == HX_TIPS_CODES && a.drop_mode == 0),
              HX_ERR_UNSUPPORTED, "post-order walk: drop mode without the register depths it assumes");
   if (tips == HX_TIPS_CODES) {
-    const size_t npad = (size_t)a.ntiles * 64, total = (size_t)a.T * npad;
-    k_pad_codes<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(a.codes, a.codes_pad, a.T, a.N,
-                                                                 (int)npad);
+    const size_t npad = (size_t)a.ntiles * 64;
+    HX_REQUIRE(a.T <= 65535, HX_ERR_UNSUPPORTED, "post-order walk: too many taxa");
+    k_pad_codes<<<dim3((unsigned)((npad / 4 + 255) / 256), (unsigned)a.T), 256, 0, st>>>(
+        a.codes, a.codes_pad, a.T, a.N, (int)npad);
   }
   dim3 grid(a.ntiles, (a.K + W - 1) / W);
   const int4* pg = reinterpret_cast<const int4*>(prog);
@@ -1142,21 +1150,25 @@ int launch_pre_dna(const PassArgs<R>& a, const int32_t* nodes, int width, int ti
 
 // The pre-order walk's 16-bit codes [T + ncher][np]: row t < T is tip t's code, row T + j
 // cherry j's two codes (first tip's | second's << 8); padding patterns are 0.
+// grid (ceil(np / 1024), T + ncher): thread j4 writes patterns 4 j4 .. 4 j4 + 3 of row
+// blockIdx.y as one 64-bit store (np is a multiple of 64, so the stores are aligned)
 __global__ void k_pair_codes(const uint8_t* __restrict__ codes, const int32_t* __restrict__ cher,
                              uint16_t* __restrict__ out, int T, int ncher, int N, int np) {
-  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (size_t)(T + ncher) * np) return;
-  const int r = (int)(idx / np), j = (int)(idx % np);
-  uint16_t v = 0;
-  if (j < N) {
-    if (r < T) {
-      v = codes[(size_t)r * N + j];
-    } else {
-      const int t0 = cher[2 * (r - T)], t1 = cher[2 * (r - T) + 1];
-      v = (uint16_t)(codes[(size_t)t0 * N + j] | (codes[(size_t)t1 * N + j] << 8));
+  const int r = blockIdx.y;
+  const int j = 4 * (blockIdx.x * blockDim.x + threadIdx.x);
+  if (j >= np) return;
+  const bool tip = r < T;
+  const uint8_t* s0 = codes + (size_t)(tip ? r : cher[2 * (r - T)]) * N;
+  const uint8_t* s1 = codes + (size_t)(tip ? r : cher[2 * (r - T) + 1]) * N;
+  uint64_t v = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (j + q < N) {
+      const uint32_t c = tip ? (uint32_t)s0[j + q] : ((uint32_t)s0[j + q] | ((uint32_t)s1[j + q] << 8));
+      v |= (uint64_t)c << (16 * q);
     }
   }
-  out[idx] = v;
+  *reinterpret_cast<uint64_t*>(out + (size_t)r * np + j) = v;
 }
 
 int launch_pre_walk(const PassArgs<float>& a, const int32_t* prog, int nprog, int tips,
@@ -1167,9 +1179,9 @@ int launch_pre_walk(const PassArgs<float>& a, const int32_t* prog, int nprog, in
   if (tips == HX_TIPS_CODES) {
     HX_REQUIRE(a.pcodes != nullptr && 2 * a.ncher <= a.T, HX_ERR_UNSUPPORTED,
                "pre-order walk: no room for the 16-bit codes");
-    const size_t total = (size_t)(a.T + a.ncher) * a.np;
-    k_pair_codes<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(a.codes, a.cher, a.pcodes, a.T,
-                                                                  a.ncher, a.N, a.np);
+    HX_REQUIRE(a.T + a.ncher <= 65535, HX_ERR_UNSUPPORTED, "pre-order walk: too many code rows");
+    k_pair_codes<<<dim3((unsigned)((a.np / 4 + 255) / 256), (unsigned)(a.T + a.ncher)), 256, 0,
+                   st>>>(a.codes, a.cher, a.pcodes, a.T, a.ncher, a.N, a.np);
     HX_HIP_TRY(hipGetLastError());
   }
   const size_t lds = (size_t)W * 2 * 16 * kXS * sizeof(float) + (size_t)4 * W * 64 * sizeof(float) +
