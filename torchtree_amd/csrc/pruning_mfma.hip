@@ -854,14 +854,18 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
   constexpr int KB = SSh::KB, NB = SSh::NB;
   constexpr bool kPReg = MT <= 2;
   constexpr int SP = 16 * MT;
+  // LDS row stride of the [16][SP] tiles: at 32 padded states, SP + 4 starts rows 4g + r of the
+  // accumulator layout 16g banks apart (SP puts the four lane groups g on the same banks); at 64
+  // the extra address registers would cost the kernel spills, so the tiles stay dense
+  constexpr int LS = MT <= 2 ? SP + 4 : SP;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int C = a.C;
-  R* Yx = reinterpret_cast<R*>(smem_raw);  // [2][C][16][SP]: P L, then Ut transposes
-  R* dens = Yx + 2 * C * kTile * SP;       // [16][4]: per row, one entry per category
+  R* Yx = reinterpret_cast<R*>(smem_raw);  // [2][C][16][LS]: P L, then Ut transposes
+  R* dens = Yx + 2 * C * kTile * LS;       // [16][4]: per row, one entry per category
   R* mxx = dens + 4 * kTile;               // [2][16][4]
-  R* Lx = mxx + 8 * kTile;                 // [2][C][16][SP]: the child's partial tile
+  R* Lx = mxx + 8 * kTile;                 // [2][C][16][LS]: the child's partial tile
   constexpr int kPtS = SP + 1;             // row stride of the P tiles (no bank conflicts)
-  R* PtL = Lx + 2 * C * kTile * SP;        // [2][C][4 KV][SP + 1]: the transpose's P (MT <= 2)
+  R* PtL = Lx + 2 * C * kTile * LS;        // [2][C][4 KV][SP + 1]: the transpose's P (MT <= 2)
   uint8_t* ctile = reinterpret_cast<uint8_t*>(PtL + (MT <= 2 ? 2 * C * 4 * KV * kPtS : 0));
   // [2][tpc * 16] codes
 
@@ -919,12 +923,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
   for (int i = 0; i < MT; ++i)
 #pragma unroll
     for (int j = 0; j < MT; ++j) G[i][j] = vec4_t<R>{R(0), R(0), R(0), R(0)};
-  R* mine = Yx + (size_t)(h * C + c) * kTile * SP;
-  const R* other = Yx + (size_t)((1 - h) * C + c) * kTile * SP;
-  R* lt = Lx + (size_t)(h * C + c) * kTile * SP;  // this wave's L tile [16 patterns][SP]
+  R* mine = Yx + (size_t)(h * C + c) * kTile * LS;
+  const R* other = Yx + (size_t)((1 - h) * C + c) * kTile * LS;
+  R* lt = Lx + (size_t)(h * C + c) * kTile * LS;  // this wave's L tile [16 patterns][LS]
   if (4 * KV < SP) {
     // states 4 KV .. SP - 1 of the tile are never written
-    for (int q = lane; q < kTile * SP; q += 64) lt[q] = R(0);
+    for (int q = lane; q < kTile * LS; q += 64) lt[q] = R(0);
   }
   // categories C .. 3 of dens and mxx stay zero, so sums and maxima over a
   // row read all four entries
@@ -980,8 +984,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
     for (int t = 0; t < MT; ++t) acc[t] = vec4_t<R>{R(0), R(0), R(0), R(0)};
 #pragma unroll
     for (int q = 0; q < KV; ++q) {
-      if constexpr (SPLIT) lt[nn * SP + g * KV + q] = __builtin_bit_cast(float, bf_pair(arow[q]));
-      else lt[nn * SP + g * KV + q] = arow[q];
+      if constexpr (SPLIT) lt[nn * LS + g * KV + q] = __builtin_bit_cast(float, bf_pair(arow[q]));
+      else lt[nn * LS + g * KV + q] = arow[q];
     }
     if constexpr (kPReg) {
       fp.product(arow, acc);
@@ -1016,7 +1020,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
     for (int t = 0; t < MT; ++t)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) mine[acc_row<R>(g, r) * SP + 16 * t + nn] = acc[t][r];
+      for (int r = 0; r < 4; ++r) mine[acc_row<R>(g, r) * LS + 16 * t + nn] = acc[t][r];
     // the next tile's child rows (the last tile re-reads its own, unused)
     pre_rows<R, KV, TIPS>(a, internal, cslab, chh, mycodes, min(tile + 1, t1 - 1), t0, nn, g, S, CS,
                           cb, arow);
@@ -1030,7 +1034,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
       R d = R(0), um = R(0);
 #pragma unroll
       for (int t = 0; t < MT; ++t) {
-        u[t][r] = up[t][r] * other[m * SP + 16 * t + nn];
+        u[t][r] = up[t][r] * other[m * LS + 16 * t + nn];
         d = fma(u[t][r], acc[t][r], d);
         um = fmax_(um, u[t][r]);
       }
@@ -1062,7 +1066,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
         for (int ks = 0; ks < 4; ++ks)
           pack_set_pair(bp[nt], ks, __builtin_bit_cast(
-                                        uint32_t, lt[acc_row<R>(g, ks) * SP + 16 * nt + nn]));
+                                        uint32_t, lt[acc_row<R>(g, ks) * LS + 16 * nt + nn]));
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) {
         BfPack<4> ap;
@@ -1076,7 +1080,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
       for (int ks = 0; ks < 4; ++ks) {
         R bv[MT];
 #pragma unroll
-        for (int nt = 0; nt < MT; ++nt) bv[nt] = lt[acc_row<R>(g, ks) * SP + 16 * nt + nn];
+        for (int nt = 0; nt < MT; ++nt) bv[nt] = lt[acc_row<R>(g, ks) * LS + 16 * nt + nn];
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) {
           const R av = coef[ks] * u[mt][ks];
@@ -1089,7 +1093,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
     for (int t = 0; t < MT; ++t)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) mine[acc_row<R>(g, r) * SP + 16 * t + nn] = u[t][r];
+      for (int r = 0; r < 4; ++r) mine[acc_row<R>(g, r) * LS + 16 * t + nn] = u[t][r];
     vec4_t<R> nu[MT];
     if (internal) {
 #pragma unroll
@@ -1101,7 +1105,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
         BfPack<KB> ap[NB];
 #pragma unroll
         for (int q = 0; q < NB * KB; ++q)
-          pack_set(ap[q / KB], q % KB, q < KV ? mine[nn * SP + g * KV + q] : 0.f);
+          pack_set(ap[q / KB], q % KB, q < KV ? mine[nn * LS + g * KV + q] : 0.f);
 #pragma unroll
         for (int t = 0; t < MT; ++t)
 #pragma unroll
@@ -1128,7 +1132,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
       } else if constexpr (kPReg) {
 #pragma unroll
         for (int ks = 0; ks < KV; ++ks) {
-          const R av = mine[nn * SP + g * KV + ks];  // K index (ks, g) = state g KV + ks
+          const R av = mine[nn * LS + g * KV + ks];  // K index (ks, g) = state g KV + ks
 #pragma unroll
           for (int t = 0; t < MT; ++t) nu[t] = mma(av, ptw[(g * KV + ks) * kPtS + 16 * t + nn], nu[t]);
         }
@@ -1139,7 +1143,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
           for (int q = 0; q < KV; q += 4) {
             const vec4_t<R> pv = *reinterpret_cast<const vec4_t<R>*>(ppt + 16 * t * 64 + q);
 #pragma unroll
-            for (int e = 0; e < 4; ++e) nu[t] = mma(mine[nn * SP + g * KV + q + e], pv[e], nu[t]);
+            for (int e = 0; e < 4; ++e) nu[t] = mma(mine[nn * LS + g * KV + q + e], pv[e], nu[t]);
           }
       }
     }
@@ -1203,7 +1207,9 @@ static size_t mfma_lds(int C, int S, bool pre, int tiles_per_chunk) {
   if (pre && (MT <= 2 || rb == 4)) {
     const int KV = MT == 2 && S <= 20 ? 5 : 4 * MT;
     const size_t pt = MT <= 2 ? 2 * (size_t)C * 4 * KV * (SP + 1) : 0;  // transpose P tiles
-    return (4 * C * kTile * SP + 12 * kTile + pt) * rb + 64 + 2 * (size_t)tiles_per_chunk * kTile;
+    // the [16][SP] tiles are stored with row stride SP + 4 at MT <= 2 (k_pre_mfma_r)
+    return (4 * C * kTile * (MT <= 2 ? SP + 4 : SP) + 12 * kTile + pt) * rb + 64 +
+           2 * (size_t)tiles_per_chunk * kTile;
   }
   if (pre) return (4 * C * kTile * SP + 3 * C * kTile) * rb + 64;
   return (C * kTile * SP + C * kTile) * rb + (size_t)(C * kTile + C * SP) * 8 + 3 * kTile * 4 + 64;
