@@ -454,10 +454,13 @@ template <typename R, int MT, int KV, int TIPS, bool SPLIT = false>
 __global__ __launch_bounds__(512) void k_post_mfma_r(PassArgs<R> a,
                                                      const int32_t* __restrict__ nodes, int S) {
   constexpr int SP = 16 * MT;
+  // LDS row stride of the product tiles: rows 4g + r of the accumulator layout start 16g banks
+  // apart (SP would put the four lane groups g on the same banks)
+  constexpr int LS = SP + 4;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int C = a.C;
-  R* Yx = reinterpret_cast<R*>(smem_raw);              // [C][16][SP] child-1 products
-  R* mx = Yx + C * kTile * SP;                         // [16][4] per-row category maxima
+  R* Yx = reinterpret_cast<R*>(smem_raw);              // [C][16][LS] child-1 products
+  R* mx = Yx + C * kTile * LS;                         // [16][4] per-row category maxima
   int* Et = reinterpret_cast<int*>(mx + 4 * kTile);    // [2 parity][2 children][16]
   uint8_t* ctile = reinterpret_cast<uint8_t*>(Et + 4 * kTile);  // [2][tpc * 16] codes
 
@@ -535,7 +538,7 @@ __global__ __launch_bounds__(512) void k_post_mfma_r(PassArgs<R> a,
 #pragma unroll
       for (int t = 0; t < MT; ++t)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) Yx[(c * kTile + acc_row<R>(g, r)) * SP + 16 * t + nn] = acc[t][r];
+        for (int r = 0; r < 4; ++r) Yx[(c * kTile + acc_row<R>(g, r)) * LS + 16 * t + nn] = acc[t][r];
     }
     int* et = Et + (tile & 1) * 2 * kTile;
     if (wave == 0 && lane < 2 * kTile) et[lane] = (int)sv;
@@ -547,7 +550,7 @@ __global__ __launch_bounds__(512) void k_post_mfma_r(PassArgs<R> a,
         R m_ = R(0);
 #pragma unroll
         for (int t = 0; t < MT; ++t) {
-          x[t][r] = acc[t][r] * Yx[(c * kTile + acc_row<R>(g, r)) * SP + 16 * t + nn];
+          x[t][r] = acc[t][r] * Yx[(c * kTile + acc_row<R>(g, r)) * LS + 16 * t + nn];
           m_ = fmax_(m_, x[t][r]);
         }
         m_ = rmax16(m_);
@@ -1249,7 +1252,7 @@ static int launch_mfma_t(bool post, const PassArgs<R>& a, const int32_t* nodes, 
   else HX_MF(k_post_mfma, MTV, KVV, HX_TIPS_DENSE, true);
   if (post && !root) {
     // scale tiles, per-row maxima, the chunk's codes
-    const size_t lds_r = (size_t)a.C * kTile * 16 * MT * sizeof(R) + 4 * kTile * sizeof(R) +
+    const size_t lds_r = (size_t)a.C * kTile * (16 * MT + 4) * sizeof(R) + 4 * kTile * sizeof(R) +
                          4 * kTile * 4 + 2 * (size_t)a.tiles_per_chunk * kTile + 64;
     HX_REQUIRE(lds_r <= 64 * 1024, HX_ERR_UNSUPPORTED, "dense-state post-order LDS exceeds 64 KiB");
     // MT 4: C <= 4, so C x 16 x 64 products take at most 16 KiB (32 KiB in FP64)
