@@ -117,8 +117,10 @@ __global__ void k_reduce_tau(const double* __restrict__ gtau, double* __restrict
   dtau[idx] = t;
 }
 
-// P and P^T of every (sample, branch, category) into zero-padded 64 x 64 tiles:
-// ppad[kbc][0][i][j] = P[i][j], ppad[kbc][1][j][i] = P[i][j]. One block per matrix.
+// P and P^T of every (sample, branch, category), zero-padded to 64 x 64, in the order the
+// 64 lanes of a dense pre-order wave read them: part 0 holds M = P, part 1 holds M = P^T, and
+// M[16 t + n][16 g + 4 w + e] sits at ppad[kbc][part][t][w][lane n + 16 g][e], so that each
+// 16-byte load of a wave (fixed t, w) reads 1 KiB of consecutive bytes. One block per matrix.
 // With `split` (FP32 only) each element is stored as the 32-bit word holding its BF16 hi
 // (low half) and lo (high half), the operand form of the split-precision products.
 __device__ __forceinline__ uint32_t pad_pair(float x) {
@@ -134,7 +136,9 @@ __global__ __launch_bounds__(256) void k_pad_mats(const R* __restrict__ mats, R*
   const R* src = mats + m * S * S;
   R* dst = ppad + m * 2 * 64 * 64;
   for (int e = threadIdx.x; e < 64 * 64; e += 256) {
-    const int i = e >> 6, j = e & 63;
+    // e = ((t 4 + w) 64 + lane) 4 + (j & 3)
+    const int lane = (e >> 2) & 63, t = e >> 10, w = (e >> 8) & 3;
+    const int i = 16 * t + (lane & 15), j = 16 * (lane >> 4) + 4 * w + (e & 3);
     R p = (i < S && j < S) ? src[i * S + j] : R(0);
     R pt = (i < S && j < S) ? src[j * S + i] : R(0);
     if constexpr (sizeof(R) == 4) {

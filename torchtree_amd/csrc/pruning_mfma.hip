@@ -33,6 +33,9 @@ namespace {
 
 template <typename R>
 using vec4_t = R __attribute__((ext_vector_type(4)));
+// four floats at a 4-byte aligned address (one 16-byte load)
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef f32x4_t f32x4u_t __attribute__((aligned(4)));
 constexpr int kTile = 16;
 
 // One 16x16x4 product D += A B on the matrix cores; FP32 and FP64 use the
@@ -818,6 +821,20 @@ __device__ __forceinline__ void pre_rows(const PassArgs<R>& a, bool internal, co
                                          R (&arow)[KV]) {
   const int nbase = tile * kTile;
   const bool nok = nbase + nn < a.N;
+  if constexpr (sizeof(R) == 4 && KV % 4 == 0) {
+    // workspace rows: KV / 4 loads of 16 bytes (4-byte aligned: a category's row starts at
+    // c S floats), no clamp needed
+    if (internal) {
+      const R* rp = cslab + (uint32_t)nbase * (uint32_t)CS + cb;
+#pragma unroll
+      for (int q = 0; q < KV; q += 4) {
+        const f32x4u_t v = *reinterpret_cast<const f32x4u_t*>(rp + q);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) arow[q + e] = v[e] * ((nok && g * KV + q + e < S) ? R(1) : R(0));
+      }
+      return;
+    }
+  }
   const R* bp;
   uint32_t base, lim;
   if (internal) {
@@ -915,11 +932,19 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
         else ptw[j * kPtS + i] = v;
       }
   }
-  // padded copies: [0][i][j] = P[i][j], [1][j][i] = P[i][j]; lane (nn, g) reads row
-  // 16 t + nn, states g KV .. g KV + 15
-  const R* ppf = kPReg ? nullptr
-                       : a.ppad + (((size_t)k * a.B + chh) * C + c) * 2 * 64 * 64 + nn * 64 + g * KV;
-  const R* ppt = ppf + 64 * 64;
+  // padded copies of P ([0]) and P^T ([1]) in lane order: lane (nn, g) finds M[16 t + nn][g KV
+  // + s .. s + 3] of either at its own 16 bytes of block 4 t + s / 4 (256 floats a block)
+  // + 4 * lane; read as one raw buffer (a lane offset, and the block offset as a scalar)
+  // (the matrix index made provably wave-uniform, so the descriptor stays in scalar registers)
+  const R* ppb = kPReg ? nullptr
+                       : a.ppad + (size_t)__builtin_amdgcn_readfirstlane((k * a.B + chh) * C + c) *
+                                      (2 * 64 * 64);
+  const __amdgpu_buffer_rsrc_t prs = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<R*>(ppb), (short)0, kPReg ? 0 : 2 * 64 * 64 * (int)sizeof(R), 0x00020000);
+  // 16 bytes of block `blk` of copy `part` (0: P, 1: P^T)
+  auto pp4 = [&](int part, int blk) -> u32x4_t {
+    return __builtin_amdgcn_raw_buffer_load_b128(prs, 16 * lane, (16 * part + blk) * 1024, 0);
+  };
   const R pc = a.props[(size_t)k * a.props_ks + c];
   vec4_t<R> G[MT][MT];  // G[Mt][Nt] lane (nn, g) reg r = dlogL/dP[16 Mt + acc_row(g, r)][16 Nt + nn]
 #pragma unroll
@@ -1004,7 +1029,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
           BfPack<8> bp;
 #pragma unroll
           for (int q = 0; q < 8; q += 4) {
-            const u32x4_t pv = load_pairs4(ppf + 16 * t * 64 + 8 * bb + q);
+            const u32x4_t pv = pp4(0, 4 * t + 2 * bb + q / 4);
 #pragma unroll
             for (int e = 0; e < 4; ++e) pack_set_pair(bp, q + e, pv[e]);
           }
@@ -1015,7 +1040,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
       for (int t = 0; t < MT; ++t)
 #pragma unroll
         for (int q = 0; q < KV; q += 4) {
-          const vec4_t<R> pv = *reinterpret_cast<const vec4_t<R>*>(ppf + 16 * t * 64 + q);
+          const vec4_t<R> pv = __builtin_bit_cast(vec4_t<R>, pp4(0, 4 * t + q / 4));
 #pragma unroll
           for (int e = 0; e < 4; ++e) acc[t] = mma(arow[q + e], pv[e], acc[t]);
         }
@@ -1125,7 +1150,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
             } else {
 #pragma unroll
               for (int q = 0; q < KB; q += 4) {
-                const u32x4_t pv = load_pairs4(ppt + 16 * t * 64 + bb * KB + q);
+                const u32x4_t pv = pp4(1, 4 * t + (bb * KB + q) / 4);
 #pragma unroll
                 for (int e = 0; e < 4; ++e) pack_set_pair(bp, q + e, pv[e]);
               }
@@ -1144,7 +1169,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
         for (int t = 0; t < MT; ++t)
 #pragma unroll
           for (int q = 0; q < KV; q += 4) {
-            const vec4_t<R> pv = *reinterpret_cast<const vec4_t<R>*>(ppt + 16 * t * 64 + q);
+            const vec4_t<R> pv = __builtin_bit_cast(vec4_t<R>, pp4(1, 4 * t + q / 4));
 #pragma unroll
             for (int e = 0; e < 4; ++e) nu[t] = mma(mine[nn * LS + g * KV + q + e], pv[e], nu[t]);
           }
