@@ -217,12 +217,13 @@ __global__ void k_pgrad_rates(int K, int B, int C, const double* __restrict__ bl
   if (threadIdx.x == 0) drates[blockIdx.x] = t;
 }
 
-// X[e][mn] = sum over the samples of e, branches and categories of xpart[row][mn], in two
-// passes with a fixed order. Pass 1, grid (NB, E): block b sums its share of e's rows into
-// part[e][b][mn]; its 256 threads read consecutive elements of the row-major [rows][SS] array:
-// with SS dividing 256, thread t keeps column t % SS of every (256 / SS)-th row from row
-// t / SS, and the threads of one column are then added in LDS; otherwise thread t keeps
-// columns t, t + 256, ... of every row. Pass 2 adds the NB parts in order.
+// X[e][mn] = sum over the samples of e, branches and categories of xpart[row][mn], with a
+// fixed order. For S * S dividing 256 (four states) and room for at least 8 parts: two passes.
+// Pass 1, grid (NB, E): block b sums its share of e's rows into part[e][b][mn]; its 256 threads
+// read consecutive elements of the row-major [rows][SS] array, thread t keeping column t % SS
+// of every (256 / SS)-th row from row t / SS, and the threads of one column are then added in
+// LDS. Pass 2 adds the NB parts in order. Otherwise (dense states: S * S = 400 or 3721 entries,
+// enough blocks by themselves) one block per (e, mn), k_pgrad_xsum.
 __global__ void k_pgrad_xsum_part(int SS, int K, int B, int C, int eb, int NB,
                                   const double* __restrict__ xpart, double* __restrict__ part) {
   __shared__ double red[256];
@@ -234,24 +235,29 @@ __global__ void k_pgrad_xsum_part(int SS, int K, int B, int C, int eb, int NB,
   const size_t lo = r0 + (size_t)b * rpb, hi = r0 + min(rows, (size_t)(b + 1) * rpb);
   double* out = part + ((size_t)e * NB + b) * SS;
   const int t = threadIdx.x;
-  if (256 % SS == 0) {
-    const int col = t % SS, step = 256 / SS;
-    double acc = 0.0;
-    for (size_t r = lo + t / SS; r < hi; r += step) acc += xpart[r * SS + col];
-    red[t] = acc;
-    __syncthreads();
-    if (t < SS) {
-      double v = 0.0;
-      for (int q = 0; q < step; ++q) v += red[q * SS + t];
-      out[t] = v;
-    }
-  } else {
-    for (int col = t; col < SS; col += 256) {
-      double acc = 0.0;
-      for (size_t r = lo; r < hi; ++r) acc += xpart[r * SS + col];
-      out[col] = acc;
-    }
+  const int col = t % SS, step = 256 / SS;
+  double acc = 0.0;
+  for (size_t r = lo + t / SS; r < hi; r += step) acc += xpart[r * SS + col];
+  red[t] = acc;
+  __syncthreads();
+  if (t < SS) {
+    double v = 0.0;
+    for (int q = 0; q < step; ++q) v += red[q * SS + t];
+    out[t] = v;
   }
+}
+
+__global__ void k_pgrad_xsum(int SS, int K, int B, int C, int eb, const double* __restrict__ xpart,
+                             double* __restrict__ X) {
+  __shared__ double red[256];
+  const int e = blockIdx.x / SS, mn = blockIdx.x % SS;
+  const size_t per = (size_t)B * C;
+  const size_t r0 = eb ? (size_t)e * per : 0;
+  const size_t r1 = eb ? r0 + per : (size_t)K * per;
+  double t = 0.0;
+  for (size_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) t += xpart[r * SS + mn];
+  t = block_reduce_sum(t, red);
+  if (threadIdx.x == 0) X[blockIdx.x] = t;
 }
 
 __global__ void k_pgrad_xsum_fin(int SS, int E, int NB, const double* __restrict__ part,
@@ -399,18 +405,19 @@ int hx_pmatrix_grad(int S, int K, int B, int C, int eigen_batched, const double*
   if (out_drates)
     hx::k_pgrad_rates<<<(unsigned)(K * C), 256, 0, st>>>(K, B, C, bls, dtau, out_drates);
   if (out_dq) {
-    // the parts go into dtau's room (n doubles), which the launches above have consumed; with
-    // no room for one part per eigensystem, one block per eigensystem writes X directly
+    // the two-pass parts go into dtau's room (n doubles), which the launches above have consumed
     const int E = eigen_batched ? K : 1;
     const int SS = S * S;
     const size_t cap = n / ((size_t)E * SS);
-    const int NB = cap >= 1 ? (int)(cap < 64 ? cap : 64) : 1;
-    double* part = cap >= 1 ? dtau : X;
-    hx::k_pgrad_xsum_part<<<dim3((unsigned)NB, (unsigned)E), 256, 0, st>>>(SS, K, B, C, eigen_batched,
-                                                                         NB, xpart, part);
-    if (part != X)
+    if (256 % SS == 0 && cap >= 8) {
+      const int NB = (int)(cap < 64 ? cap : 64);
+      hx::k_pgrad_xsum_part<<<dim3((unsigned)NB, (unsigned)E), 256, 0, st>>>(
+          SS, K, B, C, eigen_batched, NB, xpart, dtau);
       hx::k_pgrad_xsum_fin<<<(unsigned)((E * (size_t)SS + 255) / 256), 256, 0, st>>>(SS, E, NB,
-                                                                                     part, X);
+                                                                                     dtau, X);
+    } else {
+      hx::k_pgrad_xsum<<<(unsigned)(E * SS), 256, 0, st>>>(SS, K, B, C, eigen_batched, xpart, X);
+    }
     const size_t sm2 = 2 * (size_t)S * S * sizeof(double);
     hx::k_pgrad_q<<<E, 256, sm2, st>>>(S, left, right, X, out_dq);
   }

@@ -121,13 +121,13 @@ __global__ void k_reduce_tau(const double* __restrict__ gtau, double* __restrict
 // 64 lanes of a dense pre-order wave read them: part 0 holds M = P, part 1 holds M = P^T, and
 // M[16 t + n][16 g + 4 w + e] sits at ppad[kbc][part][t][w][lane n + 16 g][e], so that each
 // 16-byte load of a wave (fixed t, w) reads 1 KiB of consecutive bytes. One block per matrix.
-// With `split` (FP32 only) each element is stored as the 32-bit word holding its BF16 hi
-// (low half) and lo (high half), the operand form of the split-precision products.
-__device__ __forceinline__ uint32_t pad_pair(float x) {
+// With `split` (FP32 only) blocks w = 2 b and 2 b + 1 instead hold the BF16 hi and lo halves
+// (hi = bf16(x), lo = bf16(x - hi)) of M[16 t + n][16 g + 8 b + m], m = 0..7, as 8 BF16 values
+// each: the two operand registers of a split-precision product, loaded as they are used.
+__device__ __forceinline__ uint32_t pad_half(float x, bool lo) {
   const __bf16 hi = (__bf16)x;
-  const __bf16 lo = (__bf16)(x - (float)hi);
-  return (uint32_t)__builtin_bit_cast(uint16_t, hi) |
-         ((uint32_t)__builtin_bit_cast(uint16_t, lo) << 16);
+  const __bf16 h = lo ? (__bf16)(x - (float)hi) : hi;
+  return (uint32_t)__builtin_bit_cast(uint16_t, h);
 }
 template <typename R>
 __global__ __launch_bounds__(256) void k_pad_mats(const R* __restrict__ mats, R* __restrict__ ppad,
@@ -135,18 +135,31 @@ __global__ __launch_bounds__(256) void k_pad_mats(const R* __restrict__ mats, R*
   const size_t m = blockIdx.x;
   const R* src = mats + m * S * S;
   R* dst = ppad + m * 2 * 64 * 64;
+  auto at = [&](int i, int j, bool tr) -> R {
+    return (i < S && j < S) ? (tr ? src[j * S + i] : src[i * S + j]) : R(0);
+  };
   for (int e = threadIdx.x; e < 64 * 64; e += 256) {
-    // e = ((t 4 + w) 64 + lane) 4 + (j & 3)
-    const int lane = (e >> 2) & 63, t = e >> 10, w = (e >> 8) & 3;
-    const int i = 16 * t + (lane & 15), j = 16 * (lane >> 4) + 4 * w + (e & 3);
-    R p = (i < S && j < S) ? src[i * S + j] : R(0);
-    R pt = (i < S && j < S) ? src[j * S + i] : R(0);
+    // e = ((t 4 + w) 64 + lane) 4 + word
+    const int lane = (e >> 2) & 63, t = e >> 10, w = (e >> 8) & 3, wd = e & 3;
+    const int i = 16 * t + (lane & 15), g = lane >> 4;
+    R p, pt;
     if constexpr (sizeof(R) == 4) {
       if (split) {
-        p = __builtin_bit_cast(float, pad_pair(p));
-        pt = __builtin_bit_cast(float, pad_pair(pt));
+        // word wd of block w: elements m = 2 wd, 2 wd + 1 of pack w / 2, half w & 1
+        const int j = 16 * g + 8 * (w >> 1) + 2 * wd;
+        const bool lo = w & 1;
+        p = __builtin_bit_cast(float, pad_half(at(i, j, false), lo) |
+                                          (pad_half(at(i, j + 1, false), lo) << 16));
+        pt = __builtin_bit_cast(float, pad_half(at(i, j, true), lo) |
+                                           (pad_half(at(i, j + 1, true), lo) << 16));
+        dst[e] = p;
+        dst[64 * 64 + e] = pt;
+        continue;
       }
     }
+    const int j = 16 * g + 4 * w + wd;
+    p = at(i, j, false);
+    pt = at(i, j, true);
     dst[e] = p;
     dst[64 * 64 + e] = pt;
   }

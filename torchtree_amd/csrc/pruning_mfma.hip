@@ -813,14 +813,15 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
 
 // Child rows of one tile for k_pre_mfma_r: states g KV .. g KV + KV - 1 of
 // pattern tile * 16 + nn, every load issued (clamped caller buffers, padded
-// workspace) and out-of-range values zeroed by a multiply.
+// workspace). The values are left as loaded, so that nothing waits on them
+// until pre_rows_mask, at the start of the tile that uses them, zeroes the
+// out-of-range ones by a multiply.
 template <typename R, int KV, int TIPS>
 __device__ __forceinline__ void pre_rows(const PassArgs<R>& a, bool internal, const R* cslab,
                                          int chh, const uint8_t* mycodes, int tile, int t0,
                                          int nn, int g, int S, int CS, uint32_t cb,
                                          R (&arow)[KV]) {
   const int nbase = tile * kTile;
-  const bool nok = nbase + nn < a.N;
   if constexpr (sizeof(R) == 4 && KV % 4 == 0) {
     // workspace rows: KV / 4 loads of 16 bytes (4-byte aligned: a category's row starts at
     // c S floats), no clamp needed
@@ -830,7 +831,7 @@ __device__ __forceinline__ void pre_rows(const PassArgs<R>& a, bool internal, co
       for (int q = 0; q < KV; q += 4) {
         const f32x4u_t v = *reinterpret_cast<const f32x4u_t*>(rp + q);
 #pragma unroll
-        for (int e = 0; e < 4; ++e) arow[q + e] = v[e] * ((nok && g * KV + q + e < S) ? R(1) : R(0));
+        for (int e = 0; e < 4; ++e) arow[q + e] = v[e];
       }
       return;
     }
@@ -847,9 +848,34 @@ __device__ __forceinline__ void pre_rows(const PassArgs<R>& a, bool internal, co
     base = (uint32_t)(min(nbase + nn, a.N - 1) * S + g * KV), lim = (uint32_t)(a.N * S - 1);
   }
 #pragma unroll
-  for (int q = 0; q < KV; ++q)
-    arow[q] = bp[min(base + q, lim)] * ((nok && g * KV + q < S) ? R(1) : R(0));
+  for (int q = 0; q < KV; ++q) arow[q] = bp[min(base + q, lim)];
 }
+template <typename R, int KV>
+__device__ __forceinline__ void pre_rows_mask(int N, int tile, int nn, int g, int S,
+                                              R (&arow)[KV]) {
+  const bool nok = tile * kTile + nn < N;
+#pragma unroll
+  for (int q = 0; q < KV; ++q) arow[q] *= (nok && g * KV + q < S) ? R(1) : R(0);
+}
+
+// The 16 blocks of 16 bytes per lane of one padded P copy, consumed in order
+// with D of them in flight: block b + D is issued when block b is taken.
+template <int D>
+struct PRing {
+  u32x4_t r[D];
+  template <typename F>
+  __device__ __forceinline__ void start(F& ld, int part) {
+#pragma unroll
+    for (int b = 0; b < D; ++b) r[b] = ld(part, b);
+  }
+  template <typename F>
+  __device__ __forceinline__ u32x4_t take(F& ld, int part, int b) {
+    const u32x4_t v = r[b % D];
+    if (b + D < 16) r[b % D] = ld(part, b + D);
+    return v;
+  }
+};
+constexpr int kPD = 4;
 
 // Pre-order for at most 32 padded states with both P operands in registers
 // (the forward's P^T and the transpose's P) for the whole chunk; at 64 padded
@@ -893,8 +919,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
   const int lane = threadIdx.x & 63;
   const int h = wave / C, c = wave - h * C;
   const int nn = lane & 15, g = lane >> 4;
-  const int chunk = blockIdx.x, k = blockIdx.y, p = nodes[blockIdx.z];
-  const int ch0 = a.kids[2 * p], ch1 = a.kids[2 * p + 1];
+  // node indices read once per wave into scalar registers (the slab and P descriptors
+  // below must be wave-uniform)
+  const int chunk = blockIdx.x, k = blockIdx.y;
+  const int p = __builtin_amdgcn_readfirstlane(nodes[blockIdx.z]);
+  const int ch0 = __builtin_amdgcn_readfirstlane(a.kids[2 * p]);
+  const int ch1 = __builtin_amdgcn_readfirstlane(a.kids[2 * p + 1]);
   const int chh = h ? ch1 : ch0;
   const bool internal = chh >= a.T;
   const int CS = C * S;
@@ -933,9 +963,11 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
       }
   }
   // padded copies of P ([0]) and P^T ([1]) in lane order: lane (nn, g) finds M[16 t + nn][g KV
-  // + s .. s + 3] of either at its own 16 bytes of block 4 t + s / 4 (256 floats a block)
-  // + 4 * lane; read as one raw buffer (a lane offset, and the block offset as a scalar)
-  // (the matrix index made provably wave-uniform, so the descriptor stays in scalar registers)
+  // + s .. s + 3] of either at its own 16 bytes of block 4 t + s / 4 (256 floats a block), or
+  // in the split form the hi and lo operands of states g KV + 8 b .. + 7 at blocks 4 t + 2 b
+  // and 4 t + 2 b + 1; read as one raw buffer (a lane offset, and the block offset as a
+  // scalar; the matrix index made provably wave-uniform, so the descriptor stays in scalar
+  // registers)
   const R* ppb = kPReg ? nullptr
                        : a.ppad + (size_t)__builtin_amdgcn_readfirstlane((k * a.B + chh) * C + c) *
                                       (2 * 64 * 64);
@@ -982,29 +1014,78 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
   // after its forward product, so their latency overlaps the rest of the tile
   R arow[KV];
   pre_rows<R, KV, TIPS>(a, internal, cslab, chh, mycodes, t0, t0, nn, g, S, CS, cb, arow);
+  // the parent's adjoint (zero past N and S) and the weights of a tile
+  // at 64 padded states a tile's rows of a slab are read and written through a buffer
+  // descriptor at the tile (scalar registers), with the lane's byte offset of row acc_row(g, r)
+  // in tvo[r] and 64 t bytes as the scalar offset
+  uint32_t tvo[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) tvo[r] = (ub + r * rstep) * (uint32_t)sizeof(R);
+  auto tile_rsrc = [&](const R* sl, int tl) {
+    return __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<R*>(sl + (size_t)__builtin_amdgcn_readfirstlane(tl * kTile) * CS), (short)0,
+        kTile * CS * (int)sizeof(R), 0x00020000);
+  };
+  auto load_up = [&](int tl, R (&up)[MT][4], R (&wt)[4]) {
+    const int nb = tl * kTile;
+    const uint32_t tbl = (uint32_t)nb * (uint32_t)CS;
+    if constexpr (!ROOT && !kPReg) {
+      const __amdgpu_buffer_rsrc_t urs = tile_rsrc(pslab, tl);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const bool ok = nb + acc_row<R>(g, r) < a.N;
+#pragma unroll
+        for (int t = 0; t < MT; ++t)
+          up[t][r] = __builtin_bit_cast(
+                         R, __builtin_amdgcn_raw_buffer_load_b32(urs, (int)tvo[r], 64 * t, 0)) *
+                     ((ok && stok[t]) ? R(1) : R(0));
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const bool ok = nb + acc_row<R>(g, r) < a.N;
+#pragma unroll
+        for (int t = 0; t < MT; ++t) {
+          if constexpr (ROOT) {
+            up[t][r] = ok ? upr[t] : R(0);
+          } else {
+            up[t][r] = pslab[tbl + ub + r * rstep + 16 * t] * ((ok && stok[t]) ? R(1) : R(0));
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) wt[r] = (R)a.weights[min(nb + acc_row<R>(g, r), a.N - 1)];
+  };
+  // at 64 padded states with split products the next tile's adjoint and weights are issued
+  // with its child rows, after the forward product, so that the forward's P loads are not
+  // queued behind loads from HBM issued just before them (128 taxa, 10k patterns, 61 states:
+  // 3.30 against 3.83 ms; with FP32 products, whose longer forward hides those loads, it
+  // cost time: 4.42 against 3.86 ms)
+  constexpr bool kUpAhead = !kPReg && SPLIT;
+  R upn[MT][4], wtn[4];
+  if constexpr (kUpAhead) load_up(t0, upn, wtn);
 
   for (int tile = t0; tile < t1; ++tile) {
     const int nbase = tile * kTile;
     const uint32_t tb = (uint32_t)nbase * (uint32_t)CS;
-    // the parent's adjoint, then the weights (the child rows were issued by the
-    // previous tile)
+    pre_rows_mask<R, KV>(a.N, tile, nn, g, S, arow);
     bool rok[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) rok[r] = nbase + acc_row<R>(g, r) < a.N;
-    R up[MT][4];
+    R up[MT][4], wt[4];
+    if constexpr (kUpAhead) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
+      for (int t = 0; t < MT; ++t)
 #pragma unroll
-      for (int t = 0; t < MT; ++t) {
-        if constexpr (ROOT) {
-          up[t][r] = rok[r] ? upr[t] : R(0);
-        } else {
-          up[t][r] = pslab[tb + ub + r * rstep + 16 * t] * ((rok[r] && stok[t]) ? R(1) : R(0));
-        }
-      }
-    R wt[4];
+        for (int r = 0; r < 4; ++r) up[t][r] = upn[t][r];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) wt[r] = (R)a.weights[min(nbase + acc_row<R>(g, r), a.N - 1)];
+      for (int r = 0; r < 4; ++r) wt[r] = wtn[r];
+    } else {
+      // the parent's adjoint, then the weights (the child rows were issued by the
+      // previous tile)
+      load_up(tile, up, wt);
+    }
 
     // forward P_h L_h; the tile of L is kept as the statistic's B operand
     vec4_t<R> acc[MT];
@@ -1018,29 +1099,30 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
     if constexpr (kPReg) {
       fp.product(arow, acc);
     } else if constexpr (SPLIT) {
-      // 64 padded states: KV = 16 = two packs of 8; ppf holds the hi/lo pairs
+      // 64 padded states: KV = 16 = two packs of 8; blocks 4 t + 2 bb and 4 t + 2 bb + 1 of
+      // the padded copy are the hi and lo operands of pack bb
       BfPack<8> ap[2];
 #pragma unroll
       for (int q = 0; q < 16; ++q) pack_set(ap[q / 8], q % 8, arow[q]);
+      PRing<kPD> ring;
+      ring.start(pp4, 0);
 #pragma unroll
       for (int t = 0; t < MT; ++t)
 #pragma unroll
-        for (int bb = 0; bb < 2; ++bb) {  // one pack of P live at a time
+        for (int bb = 0; bb < 2; ++bb) {
           BfPack<8> bp;
-#pragma unroll
-          for (int q = 0; q < 8; q += 4) {
-            const u32x4_t pv = pp4(0, 4 * t + 2 * bb + q / 4);
-#pragma unroll
-            for (int e = 0; e < 4; ++e) pack_set_pair(bp, q + e, pv[e]);
-          }
+          bp.hi = __builtin_bit_cast(bf16x8_t, ring.take(pp4, 0, 4 * t + 2 * bb));
+          bp.lo = __builtin_bit_cast(bf16x8_t, ring.take(pp4, 0, 4 * t + 2 * bb + 1));
           acc[t] = mma3(ap[bb], bp, acc[t]);
         }
     } else {
+      PRing<kPD> ring;
+      ring.start(pp4, 0);
 #pragma unroll
       for (int t = 0; t < MT; ++t)
 #pragma unroll
         for (int q = 0; q < KV; q += 4) {
-          const vec4_t<R> pv = __builtin_bit_cast(vec4_t<R>, pp4(0, 4 * t + q / 4));
+          const vec4_t<R> pv = __builtin_bit_cast(vec4_t<R>, ring.take(pp4, 0, 4 * t + q / 4));
 #pragma unroll
           for (int e = 0; e < 4; ++e) acc[t] = mma(arow[q + e], pv[e], acc[t]);
         }
@@ -1052,6 +1134,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
     // the next tile's child rows (the last tile re-reads its own, unused)
     pre_rows<R, KV, TIPS>(a, internal, cslab, chh, mycodes, min(tile + 1, t1 - 1), t0, nn, g, S, CS,
                           cb, arow);
+    if constexpr (kUpAhead) load_up(min(tile + 1, t1 - 1), upn, wtn);
     lds_barrier();
     // Ut_h = U_p * (P_other L_other), in the accumulator layout; the row maxima
     // of Ut over (category, state) set the adjoint's rescale exponent
@@ -1134,6 +1217,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
         for (int q = 0; q < NB * KB; ++q)
           pack_set(ap[q / KB], q % KB, q < KV ? mine[nn * LS + g * KV + q] : 0.f);
+        PRing<kPReg ? 1 : kPD> ring;
+        if constexpr (!kPReg) ring.start(pp4, 1);
 #pragma unroll
         for (int t = 0; t < MT; ++t)
 #pragma unroll
@@ -1148,11 +1233,11 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
                                             : 0u);
               }
             } else {
-#pragma unroll
-              for (int q = 0; q < KB; q += 4) {
-                const u32x4_t pv = pp4(1, 4 * t + (bb * KB + q) / 4);
-#pragma unroll
-                for (int e = 0; e < 4; ++e) pack_set_pair(bp, q + e, pv[e]);
+              // KB = 8: blocks 4 t + 2 bb and 4 t + 2 bb + 1 are the hi and lo operands
+              static_assert(kPReg || KB == 8, "64 padded states: packs of 8");
+              if constexpr (!kPReg) {
+                bp.hi = __builtin_bit_cast(bf16x8_t, ring.take(pp4, 1, 4 * t + 2 * bb));
+                bp.lo = __builtin_bit_cast(bf16x8_t, ring.take(pp4, 1, 4 * t + 2 * bb + 1));
               }
             }
             nu[t] = mma3(ap[bb], bp, nu[t]);
@@ -1165,11 +1250,13 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
           for (int t = 0; t < MT; ++t) nu[t] = mma(av, ptw[(g * KV + ks) * kPtS + 16 * t + nn], nu[t]);
         }
       } else {
+        PRing<kPD> ring;
+        ring.start(pp4, 1);
 #pragma unroll
         for (int t = 0; t < MT; ++t)
 #pragma unroll
           for (int q = 0; q < KV; q += 4) {
-            const vec4_t<R> pv = __builtin_bit_cast(vec4_t<R>, pp4(1, 4 * t + q / 4));
+            const vec4_t<R> pv = __builtin_bit_cast(vec4_t<R>, ring.take(pp4, 1, 4 * t + q / 4));
 #pragma unroll
             for (int e = 0; e < 4; ++e) nu[t] = mma(mine[nn * LS + g * KV + q + e], pv[e], nu[t]);
           }
@@ -1180,6 +1267,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
     // U 2^-e stays below 2 S; a power-of-two scale is exact and cancels in every
     // term of G and of the likelihood ratios below this node
     if (internal) {
+      const __amdgpu_buffer_rsrc_t crs = tile_rsrc(cslab, tile);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int m = acc_row<R>(g, r);
@@ -1189,7 +1277,14 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
         if (M > R(0) && isfinite(M)) frexp_(M, &e);
 #pragma unroll
         for (int t = 0; t < MT; ++t)
-          if (rok[r] && stok[t]) cslab[tb + ub + r * rstep + 16 * t] = ldexp_(nu[t][r], -e);
+          if (rok[r] && stok[t]) {
+            if constexpr (kPReg) {
+              cslab[tb + ub + r * rstep + 16 * t] = ldexp_(nu[t][r], -e);
+            } else {
+              __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, ldexp_(nu[t][r], -e)),
+                                                    crs, (int)tvo[r], 64 * t, 0);
+            }
+          }
       }
     }
     // no barrier: the next tile's first barrier orders this tile's reads of
