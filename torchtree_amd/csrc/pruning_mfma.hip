@@ -508,34 +508,59 @@ __global__ __launch_bounds__(512) void k_post_mfma_r(PassArgs<R> a,
   const float qmask = chq >= a.T ? 1.f : 0.f;
   lds_barrier();
 
+  // a tile's child rows, loaded as they are (rows clamped to N - 1; states past S read the
+  // row's last state, or with 16-byte loads the words after the row, which the padding after
+  // the partials keeps inside the workspace); the values past S or N are zeroed with a select
+  // when the tile starts. With split products the rows are issued one tile ahead (128 taxa,
+  // 10k patterns, 61 states: 1.79 -> 1.64 ms); the longer FP32 products hide their latency
+  // (2.01 -> 2.07 ms when issued ahead), so FP32 issues them at the tile start
+  auto post_rows = [&](int tl, R (&row)[KV]) {
+    const int ncl = min(tl * kTile + nn, a.N - 1);
+    if constexpr (sizeof(R) == 4 && KV % 4 == 0) {
+      if (internal) {
+        const R* rp = cslab + (uint32_t)(ncl * CS + c * S + g * KV);
+#pragma unroll
+        for (int q = 0; q < KV; q += 4) {
+          const f32x4u_t w = *reinterpret_cast<const f32x4u_t*>(rp + q);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) row[q + e] = w[e];
+        }
+        return;
+      }
+    }
+    const R* bp;
+    uint32_t base, last;  // first element and last state of the row read
+    if (internal) {
+      bp = cslab, base = (uint32_t)(ncl * CS + c * S);
+    } else if constexpr (TIPS == HX_TIPS_CODES) {
+      bp = a.table, base = (uint32_t)(mycodes[(tl - t0) * kTile + nn] * S);
+    } else {
+      bp = a.dense + (size_t)chh * a.N * S, base = (uint32_t)(ncl * S);
+    }
+    last = base + S - 1;
+    base += g * KV;
+#pragma unroll
+    for (int q = 0; q < KV; ++q) row[q] = bp[min(base + q, last)];
+  };
+  R arow[KV];
+  if constexpr (SPLIT) post_rows(t0, arow);
+
   for (int tile = t0; tile < t1; ++tile) {
     const int nbase = tile * kTile;
     const uint32_t tb = (uint32_t)nbase * (uint32_t)CS;
-    R arow[KV];
+    if constexpr (!SPLIT) post_rows(tile, arow);
     {
-      const int n = nbase + nn;
-      const int ncl = min(n, a.N - 1);
-      const R rowm = n < a.N ? R(1) : R(0);
-      const R* bp;
-      uint32_t base, last;  // first element and last state of the row read
-      if (internal) {
-        bp = cslab, base = (uint32_t)(ncl * CS + c * S);
-      } else if constexpr (TIPS == HX_TIPS_CODES) {
-        bp = a.table, base = (uint32_t)(mycodes[(tile - t0) * kTile + nn] * S);
-      } else {
-        bp = a.dense + (size_t)chh * a.N * S, base = (uint32_t)(ncl * S);
-      }
-      last = base + S - 1;
-      base += g * KV;
+      const bool rok = nbase + nn < a.N;
 #pragma unroll
-      for (int q = 0; q < KV; ++q)
-        arow[q] = bp[min(base + q, last)] * ((g * KV + q < S) ? rowm : R(0));
+      for (int q = 0; q < KV; ++q) arow[q] = (rok && g * KV + q < S) ? arow[q] : R(0);
     }
     const float sv = (float)sq[min(nbase + (lane & 15), a.N - 1)] * qmask;
     vec4_t<R> acc[MT];
 #pragma unroll
     for (int t = 0; t < MT; ++t) acc[t] = vec4_t<R>{R(0), R(0), R(0), R(0)};
     fp.product(arow, acc);
+    // the next tile's rows (the last tile re-reads its own, unused)
+    if constexpr (SPLIT) post_rows(min(tile + 1, t1 - 1), arow);
     // lane (nn, g): acc[t][r] = Y[m = acc_row(g, r)][i = 16 t + nn]
     if (h == 1) {
 #pragma unroll
