@@ -471,8 +471,11 @@ __global__ __launch_bounds__(512) void k_post_mfma_r(PassArgs<R> a,
   const int lane = threadIdx.x & 63;
   const int h = wave / C, c = wave - h * C;
   const int nn = lane & 15, g = lane >> 4;
-  const int chunk = blockIdx.x, k = blockIdx.y, v = nodes[blockIdx.z];
-  const int ch0 = a.kids[2 * v], ch1 = a.kids[2 * v + 1];
+  // node indices in scalar registers (the parent slab's descriptor must be wave-uniform)
+  const int chunk = blockIdx.x, k = blockIdx.y;
+  const int v = __builtin_amdgcn_readfirstlane(nodes[blockIdx.z]);
+  const int ch0 = __builtin_amdgcn_readfirstlane(a.kids[2 * v]);
+  const int ch1 = __builtin_amdgcn_readfirstlane(a.kids[2 * v + 1]);
   const int chh = h ? ch1 : ch0;
   const bool internal = chh >= a.T;
   const int CS = C * S;
@@ -587,6 +590,9 @@ __global__ __launch_bounds__(512) void k_post_mfma_r(PassArgs<R> a,
     }
     lds_barrier();
     if (h == 0) {
+      const __amdgpu_buffer_rsrc_t vrs = __builtin_amdgcn_make_buffer_rsrc(
+          vslab + (size_t)__builtin_amdgcn_readfirstlane(nbase) * CS, (short)0,
+          kTile * CS * (int)sizeof(R), 0x00020000);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int m = acc_row<R>(g, r), n = nbase + m;
@@ -598,7 +604,17 @@ __global__ __launch_bounds__(512) void k_post_mfma_r(PassArgs<R> a,
         if (n < a.N) {
 #pragma unroll
           for (int t = 0; t < MT; ++t)
-            if (stok[t]) vslab[tb + ub + r * rstep + 16 * t] = ldexp_(x[t][r], -e);
+            if (stok[t]) {
+              if constexpr (sizeof(R) == 4) {
+                // a tile's rows through a descriptor at the tile: lane offset of row
+                // acc_row(g, r) in bytes, 64 t bytes as the scalar offset
+                __builtin_amdgcn_raw_buffer_store_b32(
+                    __builtin_bit_cast(uint32_t, ldexp_(x[t][r], -e)), vrs,
+                    (int)((ub + r * rstep) * (uint32_t)sizeof(R)), 64 * t, 0);
+              } else {
+                vslab[tb + ub + r * rstep + 16 * t] = ldexp_(x[t][r], -e);
+              }
+            }
           if (c == 0 && nn == 0) a.scales[((size_t)(v - a.T) * a.K + k) * a.N + n] = E;
         }
       }
@@ -1040,9 +1056,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
   R arow[KV];
   pre_rows<R, KV, TIPS>(a, internal, cslab, chh, mycodes, t0, t0, nn, g, S, CS, cb, arow);
   // the parent's adjoint (zero past N and S) and the weights of a tile
-  // at 64 padded states a tile's rows of a slab are read and written through a buffer
-  // descriptor at the tile (scalar registers), with the lane's byte offset of row acc_row(g, r)
-  // in tvo[r] and 64 t bytes as the scalar offset
+  // in FP32 a tile's rows of a slab are read and written through a buffer descriptor at the
+  // tile (scalar registers), with the lane's byte offset of row acc_row(g, r) in tvo[r] and
+  // 64 t bytes as the scalar offset: no 64-bit address arithmetic per load or store
   uint32_t tvo[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) tvo[r] = (ub + r * rstep) * (uint32_t)sizeof(R);
@@ -1054,7 +1070,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
   auto load_up = [&](int tl, R (&up)[MT][4], R (&wt)[4]) {
     const int nb = tl * kTile;
     const uint32_t tbl = (uint32_t)nb * (uint32_t)CS;
-    if constexpr (!ROOT && !kPReg) {
+    if constexpr (!ROOT && sizeof(R) == 4) {
       const __amdgpu_buffer_rsrc_t urs = tile_rsrc(pslab, tl);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -1303,7 +1319,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
         for (int t = 0; t < MT; ++t)
           if (rok[r] && stok[t]) {
-            if constexpr (kPReg) {
+            if constexpr (sizeof(R) != 4) {
               cslab[tb + ub + r * rstep + 16 * t] = ldexp_(nu[t][r], -e);
             } else {
               __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, ldexp_(nu[t][r], -e)),
