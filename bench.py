@@ -56,6 +56,8 @@ def parse():
                     help="dense-state products of the FP32 step: FP32 matrix cores, or split "
                          "BF16 (hi + lo halves, three products, FP32 accumulation); auto: split "
                          "BF16 for the 20- and 61-state workloads, which specify it")
+    ap.add_argument("--dense-samples", type=int, default=1, metavar="K",
+                    help="samples K of --config c4a / c4b (1 by default; the served dense requests run K up to 32)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-mix measurement")
     ap.add_argument("--serve-jobs", type=int, default=200)
     ap.add_argument("--serve-requests", type=int, default=200)
@@ -346,10 +348,11 @@ def make_config(args, synthetic):
         return synthetic.config2(seed=args.seed, N=args.patterns, K=args.samples, T=args.taxa)
     if args.config == "c3":
         return synthetic.config3()
+    K = args.dense_samples
     if args.config == "c4a":
-        return synthetic.make_workload(256, 20_000, 1, model="aa", site="gamma", shape=0.5,
+        return synthetic.make_workload(256, 20_000, K, model="aa", site="gamma", shape=0.5,
                                        bl_mean=0.05, seed=41, name="config4a-aa-256taxa-20000pat")
-    return synthetic.make_workload(128, 10_000, 1, model="mg94", site="gamma", shape=0.5,
+    return synthetic.make_workload(128, 10_000, K, model="mg94", site="gamma", shape=0.5,
                                    bl_mean=0.05, seed=42, name="config4b-mg94-128taxa-10000pat")
 
 
