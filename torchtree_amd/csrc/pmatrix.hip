@@ -44,6 +44,67 @@ __global__ void k_pmatrix_elem(int S, int K, int B, int C, int eb, const double*
   out[idx] = (O)acc;
 }
 
+// Dense spectra (17..64 states): P = L diag(e) R on the FP64 matrix cores, one block of 256
+// threads per matrix. S is padded to SP = 16 MT with zeros; L diag(e) and R are staged in LDS
+// with row stride SP + 1, and 16x16 tiles are formed with v_mfma_f64_16x16x4f64 (A[i][k] from
+// lane i + 16 k, B[k][j] from lane j + 16 k, D[i][j] in lane j + 16 (i % 4), register i / 4).
+// The same values as k_pmatrix_block up to the order of the sums.
+template <typename O, int MT>
+__global__ __launch_bounds__(256) void k_pmatrix_mfma(int S, int K, int B, int C, int eb,
+                                                      const double* __restrict__ lam,
+                                                      const double* __restrict__ L,
+                                                      const double* __restrict__ R,
+                                                      const double* __restrict__ bls,
+                                                      const double* __restrict__ rates, int rb,
+                                                      O* __restrict__ out, int order) {
+  typedef double d4 __attribute__((ext_vector_type(4)));
+  constexpr int SP = 16 * MT, LD = SP + 1;
+  extern __shared__ double sm[];
+  double* Ab = sm;            // [SP][LD]: L[i][m] e[m]
+  double* Bb = sm + SP * LD;  // [SP][LD]: R[m][j]
+  double* ex = sm + 2 * SP * LD;
+  const size_t mat = blockIdx.x;
+  const int c = (int)(mat % C);
+  const size_t kb = mat / C;
+  const int b = (int)(kb % B), k = (int)(kb / B);
+  const int e = eb ? k : 0;
+  const size_t SS = (size_t)S * S;
+  const double tau = bls[(size_t)k * B + b] * rates[(size_t)(rb ? k : 0) * C + c];
+  const int tid = threadIdx.x;
+  if (tid < S) {
+    const double lm = lam[(size_t)e * S + tid];
+    ex[tid] = order ? lm * exp(lm * tau) : exp(lm * tau);
+  }
+  __syncthreads();
+  const double* le = L + (size_t)e * SS;
+  const double* re = R + (size_t)e * SS;
+  for (int q = tid; q < SP * SP; q += 256) {
+    const int i = q / SP, j = q % SP;
+    const bool in = i < S && j < S;
+    Ab[i * LD + j] = in ? le[i * S + j] * ex[j] : 0.0;
+    Bb[i * LD + j] = in ? re[i * S + j] : 0.0;
+  }
+  __syncthreads();
+  const int wave = tid >> 6, lane = tid & 63, nn = lane & 15, g4 = lane >> 4;
+  O* o = out + mat * SS;
+  for (int t = wave; t < MT * MT; t += 4) {
+    const int ti = t / MT, tj = t % MT;
+    d4 d = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ks = 0; ks < SP / 4; ++ks) {
+      const int m = 4 * ks + g4;
+      d = __builtin_amdgcn_mfma_f64_16x16x4f64(Ab[(16 * ti + nn) * LD + m],
+                                               Bb[m * LD + 16 * tj + nn], d, 0, 0, 0);
+    }
+    const int j = 16 * tj + nn;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = 16 * ti + g4 + 4 * r;
+      if (i < S && j < S) o[(size_t)i * S + j] = (O)d[r];
+    }
+  }
+}
+
 // One block per matrix for larger S: exponentials staged in LDS.
 template <typename O>
 __global__ void k_pmatrix_block(int S, int K, int B, int C, int eb, const double* __restrict__ lam,
@@ -386,6 +447,72 @@ __global__ void k_pgrad_q(int S, const double* __restrict__ L, const double* __r
   }
 }
 
+// dQ = R^T X L^T for 17..64 states on the FP64 matrix cores: one block of 256 threads per
+// eigensystem, S padded to SP = 16 MT, X and L, then R and X L^T staged in LDS (row stride
+// SP + 1); the same products as k_pgrad_q up to the order of the sums.
+template <int MT>
+__global__ __launch_bounds__(256) void k_pgrad_q_mfma(int S, const double* __restrict__ L,
+                                                      const double* __restrict__ R,
+                                                      const double* __restrict__ Xall,
+                                                      double* __restrict__ dq) {
+  typedef double d4 __attribute__((ext_vector_type(4)));
+  constexpr int SP = 16 * MT, LD = SP + 1;
+  extern __shared__ double sm[];
+  double* Ab = sm;                // [SP][LD]: X, then R
+  double* Bb = sm + SP * LD;      // [SP][LD]: L
+  double* Tb = sm + 2 * SP * LD;  // [SP][LD]: X L^T
+  const int SS = S * S;
+  const int e = blockIdx.x, tid = threadIdx.x;
+  const double* xe = Xall + (size_t)e * SS;
+  const double* le = L + (size_t)e * SS;
+  const double* re = R + (size_t)e * SS;
+  for (int q = tid; q < SP * SP; q += 256) {
+    const int i = q / SP, j = q % SP;
+    const bool in = i < S && j < S;
+    Ab[i * LD + j] = in ? xe[i * S + j] : 0.0;
+    Bb[i * LD + j] = in ? le[i * S + j] : 0.0;
+  }
+  __syncthreads();
+  const int wave = tid >> 6, lane = tid & 63, nn = lane & 15, g4 = lane >> 4;
+  // T[m][j] = sum_n X[m][n] L[j][n]
+  for (int t = wave; t < MT * MT; t += 4) {
+    const int tm = t / MT, tj = t % MT;
+    d4 d = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ks = 0; ks < SP / 4; ++ks) {
+      const int n = 4 * ks + g4;
+      d = __builtin_amdgcn_mfma_f64_16x16x4f64(Ab[(16 * tm + nn) * LD + n],
+                                               Bb[(16 * tj + nn) * LD + n], d, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) Tb[(16 * tm + g4 + 4 * r) * LD + 16 * tj + nn] = d[r];
+  }
+  __syncthreads();
+  for (int q = tid; q < SP * SP; q += 256) {
+    const int i = q / SP, j = q % SP;
+    Ab[i * LD + j] = (i < S && j < S) ? re[i * S + j] : 0.0;
+  }
+  __syncthreads();
+  // dQ[i][j] = sum_m R[m][i] T[m][j]
+  double* o = dq + (size_t)e * SS;
+  for (int t = wave; t < MT * MT; t += 4) {
+    const int ti = t / MT, tj = t % MT;
+    d4 d = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ks = 0; ks < SP / 4; ++ks) {
+      const int m = 4 * ks + g4;
+      d = __builtin_amdgcn_mfma_f64_16x16x4f64(Ab[m * LD + 16 * ti + nn],
+                                               Tb[m * LD + 16 * tj + nn], d, 0, 0, 0);
+    }
+    const int j = 16 * tj + nn;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = 16 * ti + g4 + 4 * r;
+      if (i < S && j < S) o[i * S + j] = d[r];
+    }
+  }
+}
+
 }  // namespace hx
 
 extern "C" {
@@ -412,6 +539,26 @@ static int pmatrix_impl(int S, int K, int B, int C, int eigen_batched, const dou
       hx::k_pmatrix_elem<float><<<grid, 256, 0, st>>>(S, K, B, C, eigen_batched, lambda, left,
                                                        right, bls, rates, rates_batched,
                                                        static_cast<float*>(mats), order);
+  } else if (S > 16) {
+    HX_REQUIRE(nmat < (1u << 31), HX_ERR_UNSUPPORTED, "too many matrices");
+#define HX_PM(OT, MTV)                                                                       \
+  do {                                                                                       \
+    const size_t smm = (2 * (size_t)(16 * MTV) * (16 * MTV + 1) + 16 * MTV) * sizeof(double); \
+    if (smm > 64 * 1024)                                                                     \
+      HX_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(hx::k_pmatrix_mfma<OT, MTV>), \
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)smm)); \
+    hx::k_pmatrix_mfma<OT, MTV><<<(unsigned)nmat, 256, smm, st>>>(                           \
+        S, K, B, C, eigen_batched, lambda, left, right, bls, rates, rates_batched,           \
+        static_cast<OT*>(mats), order);                                                      \
+  } while (0)
+    if (dtype == HX_F64) {
+      if (S <= 32) HX_PM(double, 2);
+      else HX_PM(double, 4);
+    } else {
+      if (S <= 32) HX_PM(float, 2);
+      else HX_PM(float, 4);
+    }
+#undef HX_PM
   } else {
     HX_REQUIRE(nmat < (1u << 31), HX_ERR_UNSUPPORTED, "too many matrices");
     if (dtype == HX_F64)
@@ -525,8 +672,22 @@ int hx_pmatrix_grad(int S, int K, int B, int C, int eigen_batched, const double*
     } else {
       hx::k_pgrad_xsum<<<(unsigned)(E * SS), 256, 0, st>>>(SS, K, B, C, eigen_batched, xpart, X);
     }
-    const size_t sm2 = 2 * (size_t)S * S * sizeof(double);
-    hx::k_pgrad_q<<<E, 256, sm2, st>>>(S, left, right, X, out_dq);
+    if (S > 16) {
+#define HX_PQ(MTV)                                                                           \
+  do {                                                                                       \
+    const size_t smm = 3 * (size_t)(16 * MTV) * (16 * MTV + 1) * sizeof(double);            \
+    if (smm > 64 * 1024)                                                                     \
+      HX_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(hx::k_pgrad_q_mfma<MTV>), \
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)smm)); \
+    hx::k_pgrad_q_mfma<MTV><<<E, 256, smm, st>>>(S, left, right, X, out_dq);                 \
+  } while (0)
+      if (S <= 32) HX_PQ(2);
+      else HX_PQ(4);
+#undef HX_PQ
+    } else {
+      const size_t sm2 = 2 * (size_t)S * S * sizeof(double);
+      hx::k_pgrad_q<<<E, 256, sm2, st>>>(S, left, right, X, out_dq);
+    }
   }
   HX_HIP_TRY(hipGetLastError());
   return HX_OK;
