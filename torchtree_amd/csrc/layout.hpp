@@ -88,7 +88,10 @@ inline Tiling make_tiling(int N, int C, int K, int S, int rb) {
     t.ppb = kWave;  // one wave per category, one lane per pattern
   } else {
     t.ppb = wide_nt(C, S, rb);
-    min_tiles = 4;
+    // FP32 at 33..64 states: chunks of at least 8 tiles, since each chunk's d/dP partials
+    // (C S^2 floats per branch) weigh as much as several tiles of partials (128 taxa, 10k
+    // patterns, 61 states: at K = 32 the step 144 -> 128 ms; at K = 1 the pre-order +2 %)
+    min_tiles = (S > 32 && rb == 4) ? 8 : 4;
   }
   t.ntiles = (N + t.ppb - 1) / t.ppb;
   if (use_pre_walk(S, C, rb, N, K)) {
