@@ -88,10 +88,12 @@ inline Tiling make_tiling(int N, int C, int K, int S, int rb) {
     t.ppb = kWave;  // one wave per category, one lane per pattern
   } else {
     t.ppb = wide_nt(C, S, rb);
-    // FP32 at 33..64 states: chunks of at least 8 tiles, since each chunk's d/dP partials
-    // (C S^2 floats per branch) weigh as much as several tiles of partials (128 taxa, 10k
-    // patterns, 61 states: at K = 32 the step 144 -> 128 ms; at K = 1 the pre-order +2 %)
-    min_tiles = (S > 32 && rb == 4) ? 8 : 4;
+    // FP32 at 17..64 states: chunks of at least 8 tiles, since each chunk's d/dP partials
+    // (C S^2 floats per branch) and its P set-up weigh as much as several tiles of partials
+    // (128 taxa, 10k patterns, 61 states: at K = 32 the step 144 -> 128 ms, at K = 1 the
+    // pre-order +2 %; the 200-request serving mix, whose smaller amino-acid slices get larger
+    // chunks: +14 % served throughput)
+    min_tiles = (S > 16 && rb == 4) ? 8 : 4;
   }
   t.ntiles = (N + t.ppb - 1) / t.ppb;
   if (use_pre_walk(S, C, rb, N, K)) {
