@@ -61,7 +61,8 @@ inline int wide_nt(int C, int S, int rb) {
 // S == 4 uses tiles of 64 patterns (a wave per category, a lane per pattern);
 // other S use tiles of wide_nt() patterns handled by (child, category, state)
 // lanes.
-// The chunking and the kernels depend on N (and the tile shape) only, never on K: a sample's
+// The chunking and the kernels depend on N, S, the precision and the tile shape only, never
+// on K: a sample's
 // sums run in the same order whatever batch it is evaluated in, so its result is bitwise
 // identical alone and in a batch.
 constexpr int kTargetChunks = 128;
